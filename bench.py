@@ -1,0 +1,154 @@
+"""Benchmark of the MI355X grid-SPH step (libsph_hip.so).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dam_break_1m]
+
+Runs W untimed steps, then times exactly K fused steps (grid + density + forces
++ walls + integrate) of the workload on one GPU with the state resident in HBM,
+and prints one JSON line: particle-steps/s, ms per step, the roofline of the
+dominant kernel (timed live with HIP events on the library's stream) and the
+CPU oracle timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "gmu-water-simulation_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per particle per stage (compulsory reads + writes of the
+# step's data flow; per-cell terms separately).
+ALG_BYTES = {
+    "hash": 16,       # pos r 12 + cell w 4
+    "sort": 8,        # cell r 4 + permutation w 4
+    "reorder": 56,    # id, pos, vel r 28 + w 28
+    "density": 16,    # pos r 12 + density w 4
+    "forces": 52,     # pos, vel, density r 28 + pos, vel w 24
+}
+ALG_BYTES_PER_CELL = 24
+
+
+def host_cpu():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(pos, box, steps):
+    """The CPU oracle (restated CCPUParticleSimulator, reference summation order,
+    single thread) on the first `steps` steps of the same workload."""
+    import oracle
+    s = oracle.OracleSim(box, len(pos))
+    s.add_particles(pos)
+    t0 = time.perf_counter()
+    s.step(steps)
+    dt = time.perf_counter() - t0
+    model, ncpu = host_cpu()
+    return {"value": len(pos) * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "steps 0-%d of the %d-particle workload, oracle in the reference's bucket order, "
+                      "1 thread (gcc -O2 -ffp-contract=off)" % (steps - 1, len(pos)),
+            "seconds": dt, "host": model, "host_threads": ncpu}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="dam_break_1m")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (a scene would be "
+              "sharded by spatial slabs with halo exchange, which is not built)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import sphhip
+    from sphhip import _lib
+
+    pos, w = sphhip.scenes.workload(args.workload)
+    n = len(pos)
+    box = w["box"]
+    print("bench: workload %s, %d particles, warmup %d, steps %d" % (args.workload, n, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+
+    sim = sphhip.HIPGridParticleSimulator(box, capacity=n, flags=0)
+    sim.upload(pos)
+    (rx, ry, rz), ncells = sim.grid_dims()
+    sim.advance(args.warmup) if args.warmup else None
+    sim.sync()
+    t0 = time.perf_counter()
+    ev_ms = sim.advance(args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = n * args.steps / wall
+
+    # Per-stage device times from HIP events on the library's stream, in a
+    # separate profiled context over the same workload (after the same warmup).
+    prof = sphhip.HIPGridParticleSimulator(box, capacity=n, flags=_lib.FLAG_PROFILE)
+    prof.upload(pos)
+    prof.advance(args.warmup) if args.warmup else None
+    prof.stage_times(reset=True)
+    prof.advance(args.profile_steps)
+    st = prof.stage_times()
+    stages = {k: (ms / cnt if cnt else 0.0) for k, (ms, cnt) in st.items()}
+    dominant = max(("density", "forces"), key=lambda k: stages[k])
+    dom_ms = stages[dominant]
+    achieved = ALG_BYTES[dominant] * n / (dom_ms * 1e-3) / 1e9
+    step_bytes = sum(ALG_BYTES.values()) * n + ALG_BYTES_PER_CELL * ncells
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_ms": dom_ms, "alg_bytes_per_particle": ALG_BYTES[dominant],
+                "step_alg_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9}
+    prof.close()
+
+    line = {
+        "metric": "particle-steps/s on 1M dam break",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "particles": n, "lattice": list(w["dims"]), "box_m": box,
+                   "grid": [rx, ry, rz], "cells": ncells, "seed": w["seed"], "parallelism": "single GPU"},
+        "device": sim.getSelectedDevice(),
+        "event_ms_per_step": ev_ms / args.steps,
+        "stage_ms": stages,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU oracle on %d steps" % args.cpu_steps, file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(pos, box, args.cpu_steps)
+    else:
+        line["cpu_baseline"] = None
+    sim.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,96 @@
+"""ctypes binding of lib/libsph_hip.so (the C ABI declared in include/sph_hip.h).
+
+The library is the product: if it is not built this module raises, it never
+falls back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsph_hip.so")
+
+# status codes (include/sph_hip.h)
+SPH_OK, SPH_ERR_INVALID, SPH_ERR_CAPACITY, SPH_ERR_HIP, SPH_ERR_OOM, SPH_ERR_NO_DEVICE, SPH_ERR_STATE, \
+    SPH_ERR_FAULT = range(8)
+PHASE_GRID, PHASE_DENSITY, PHASE_FORCES, PHASE_COLLISIONS, PHASE_INTEGRATE = range(5)
+FLAG_DIAGNOSTICS, FLAG_PROFILE = 1, 2
+(DEBUG_CELL_PRESORT, DEBUG_PERM, DEBUG_CELL, DEBUG_CELL_START, DEBUG_NEIGHBOURS, DEBUG_DENSITY,
+ DEBUG_PRESSURE, DEBUG_ACCEL, DEBUG_IDS) = range(9)
+STAGES = ["hash", "sort", "cells", "reorder", "density", "forces", "collisions", "integrate"]
+
+# Every entry point of include/sph_hip.h with its ctypes signature.
+_vp = C.c_void_p
+_f = C.c_float
+_u32 = C.c_uint32
+SIGNATURES = {
+    "sph_default_params": (None, [_vp, _f, _u32]),
+    "sph_grid_res": (C.c_int, [_f]),
+    "sph_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sph_destroy": (None, [_vp]),
+    "sph_upload": (C.c_int, [_vp, _vp, _vp, _vp, _u32]),
+    "sph_append": (C.c_int, [_vp, _vp, _vp, _vp, _u32]),
+    "sph_set_gravity": (C.c_int, [_vp, _f, _f, _f]),
+    "sph_phase": (C.c_int, [_vp, C.c_int, _vp]),
+    "sph_step": (C.c_int, [_vp, _u32, _vp]),
+    "sph_download": (C.c_int, [_vp, _vp, _vp, _vp, _u32, C.c_int]),
+    "sph_debug": (C.c_int, [_vp, C.c_int, _vp, C.c_size_t]),
+    "sph_count": (C.c_int, [_vp, C.POINTER(_u32)]),
+    "sph_grid_dims": (C.c_int, [_vp, _vp, _vp]),
+    "sph_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "sph_sync": (C.c_int, [_vp]),
+    "sph_stage_times": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "sph_last_error": (C.c_char_p, []),
+    "sph_scene_dam_break": (C.c_int, [_f, _vp, _u32, C.POINTER(_u32)]),
+    "sph_scene_lattice": (C.c_int, [_u32, _u32, _u32, _vp, _f, C.c_uint64, _vp]),
+}
+
+
+class sph_params(C.Structure):
+    _fields_ = [("box", C.c_float * 3), ("gravity", C.c_float * 3), ("dt", C.c_float),
+                ("capacity", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class SphError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("sph status %d: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load lib/libsph_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsph_hip.so is not built (%s); run `make -C %s` or "
+                           "__graft_entry__.build()" % (LIB_PATH, PKG_ROOT))
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != SPH_OK:
+        raise SphError(status, load().sph_last_error().decode(errors="replace"))
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)

@@ -1,0 +1,178 @@
+/*
+ * sph_hip.h -- C ABI of libsph_hip.so, the MI355X grid-SPH step.
+ *
+ * The library replaces the OpenCL grid path of the reference (CGPUParticleSimulator
+ * + CLWrapper + resources/kernels/sph_grid.cl, sph_common.cl) with hand-written HIP
+ * kernels for gfx950. A host class deriving from CBaseParticleSimulator
+ * (include/CBaseParticleSimulator.h:29-110) calls these entry points; its five
+ * phase virtuals map 1:1 onto sph_phase().
+ *
+ * Conventions
+ *  - Every entry point returns sph_status (0 = SPH_OK) and never throws. The text of
+ *    the last failure on the calling thread is sph_last_error(), as CLException::what()
+ *    is for the reference (include/CLWrapper.h:18-22).
+ *  - Host buffers are caller-owned; the library owns all device memory.
+ *  - Positions/velocities cross the ABI as packed xyz float triples (AoS, 12 B), the
+ *    order of CParticle::Physics' position/velocity (include/CParticle.h:25-26).
+ *  - "Array order" is the device's current particle order: stable-sorted by cell as of
+ *    the last grid phase. `ids` give each slot's particle id.
+ *  - Physics constants are the reference's compile-time values (include/CParticle.h:80-84,
+ *    include/CCollisionGeometry.h:20-21); the arithmetic follows CCPUParticleSimulator
+ *    (src/CCPUParticleSimulator.cpp) term by term in a fixed canonical summation order.
+ */
+#ifndef SPH_HIP_H
+#define SPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int sph_status;
+enum {
+    SPH_OK = 0,
+    SPH_ERR_INVALID = 1,    /* bad argument */
+    SPH_ERR_CAPACITY = 2,   /* particle count beyond the context's capacity */
+    SPH_ERR_HIP = 3,        /* a HIP runtime call failed */
+    SPH_ERR_OOM = 4,        /* device allocation failed */
+    SPH_ERR_NO_DEVICE = 5,  /* no usable gfx950 device */
+    SPH_ERR_STATE = 6,      /* phase called out of order / nothing uploaded */
+    SPH_ERR_FAULT = 7       /* injected failure (SPH_FAULT_AT_STEP env hook) */
+};
+
+/* Phases, in CBaseParticleSimulator::step order (src/CBaseParticleSimulator.cpp:116-144). */
+enum {
+    SPH_PHASE_GRID = 0,       /* updateGrid: hash, stable sort by cell, cell table, reorder */
+    SPH_PHASE_DENSITY = 1,    /* updateDensityPressure */
+    SPH_PHASE_FORCES = 2,     /* updateForces: pressure + viscosity + gravity */
+    SPH_PHASE_COLLISIONS = 3, /* updateCollisions: wall response added to the acceleration */
+    SPH_PHASE_INTEGRATE = 4   /* integrate */
+};
+
+/* Context flags. */
+enum {
+    SPH_FLAG_DIAGNOSTICS = 1u << 0, /* keep neighbour counts and accelerations for sph_debug */
+    SPH_FLAG_PROFILE = 1u << 1      /* time every stage with HIP events (sph_stage_times) */
+};
+
+/* sph_debug kinds. Per-particle arrays are in the current array order unless noted. */
+enum {
+    SPH_DEBUG_CELL_PRESORT = 0, /* uint32[n]: cell of each particle in the order before the last sort */
+    SPH_DEBUG_PERM = 1,         /* uint32[n]: pre-sort array index of the particle at sorted slot k */
+    SPH_DEBUG_CELL = 2,         /* uint32[n]: cell of each particle (sorted order) */
+    SPH_DEBUG_CELL_START = 3,   /* uint32[C+1]: first sorted slot of each cell */
+    SPH_DEBUG_NEIGHBOURS = 4,   /* uint32[n]: accepted neighbours incl. self (needs DIAGNOSTICS) */
+    SPH_DEBUG_DENSITY = 5,      /* float[n] */
+    SPH_DEBUG_PRESSURE = 6,     /* float[n]: gas_stiffness * (density - rest_density) */
+    SPH_DEBUG_ACCEL = 7,        /* float[3n] xyz (needs DIAGNOSTICS in the fused step) */
+    SPH_DEBUG_IDS = 8           /* uint32[n] */
+};
+
+/* Per-stage timer slots of sph_stage_times. */
+enum {
+    SPH_STAGE_HASH = 0, SPH_STAGE_SORT = 1, SPH_STAGE_CELLS = 2, SPH_STAGE_REORDER = 3,
+    SPH_STAGE_DENSITY = 4, SPH_STAGE_FORCES = 5, SPH_STAGE_COLLISIONS = 6, SPH_STAGE_INTEGRATE = 7,
+    SPH_STAGE_COUNT = 8
+};
+
+typedef struct sph_ctx sph_ctx;
+
+typedef struct {
+    float box[3];       /* m_boxSize (include/CBaseParticleSimulator.h:87); walls at +-box/2 */
+    float gravity[3];   /* initial gravity, (0, -9.80665, 0) in the reference */
+    float dt;           /* 0.01f in the reference (src/CBaseParticleSimulator.cpp:7) */
+    uint32_t capacity;  /* maximum particle count (m_maxParticlesCount) */
+    uint32_t flags;     /* SPH_FLAG_* */
+} sph_params;
+
+/* Fill p with the reference defaults for a cubic box of edge `box` and the given
+ * capacity (CBaseParticleSimulator ctor, src/CBaseParticleSimulator.cpp:3-36). */
+void sph_default_params(sph_params *p, float box, uint32_t capacity);
+
+/* Grid resolution (int) ceil(box / h), src/CBaseParticleSimulator.cpp:27-31. */
+int sph_grid_res(float box);
+
+/* Replaces: CGPUParticleSimulator ctor + setupKernels (src/CGPUParticleSimulator.cpp:3-55,
+ * src/CGPUBaseParticleSimulator.cpp:4-9, 39-57). Allocates all device buffers. */
+sph_status sph_create(const sph_params *params, sph_ctx **out);
+
+/* Replaces: ~CGPUParticleSimulator (include/CGPUParticleSimulator.h:17). NULL is a no-op. */
+void sph_destroy(sph_ctx *ctx);
+
+/* Replaces: the blocking H2D of m_clParticles (src/CGPUParticleSimulator.cpp:61).
+ * Sets the state to n particles (array order = the given order). ids may be NULL
+ * (then id = index). */
+sph_status sph_upload(sph_ctx *ctx, const float *pos3, const float *vel3, const uint32_t *ids, uint32_t n);
+
+/* Replaces: addParticle via generateParticles (src/CBaseParticleSimulator.cpp:67-74,
+ * 187-210), which the reference ships by re-uploading everything
+ * (src/CGPUParticleSimulator.cpp:61). Appends n particles at the end of the array
+ * order. ids may be NULL (then id = current count + index). */
+sph_status sph_append(sph_ctx *ctx, const float *pos3, const float *vel3, const uint32_t *ids, uint32_t n);
+
+/* Replaces: CGPUBaseParticleSimulator::setGravityVector (src/CGPUBaseParticleSimulator.cpp:11-15). */
+sph_status sph_set_gravity(sph_ctx *ctx, float gx, float gy, float gz);
+
+/* Replaces: updateGrid / updateDensityPressure / updateForces / updateCollisions /
+ * integrate (include/CBaseParticleSimulator.h:92-96; src/CGPUParticleSimulator.cpp:57-172,
+ * src/CGPUBaseParticleSimulator.cpp:59-97). Runs one phase on the device. If ms_out is
+ * not NULL the call waits and stores the phase's device time in ms (the reference's
+ * phase methods return ms). */
+sph_status sph_phase(sph_ctx *ctx, int phase, float *ms_out);
+
+/* Replaces: CBaseParticleSimulator::step (src/CBaseParticleSimulator.cpp:116-144), nsteps
+ * times, with forces + walls + integrate fused. Bit-identical to calling the five phases.
+ * If ms_out is not NULL the call waits and stores the device time of all nsteps in ms. */
+sph_status sph_step(sph_ctx *ctx, uint32_t nsteps, float *ms_out);
+
+/* Replaces: the readback in CGPUBaseParticleSimulator::integrate
+ * (src/CGPUBaseParticleSimulator.cpp:83-91). Copies n (= count) particles; by_id != 0
+ * writes particle id k to row k, else rows follow the array order. Any pointer may be NULL. */
+sph_status sph_download(sph_ctx *ctx, float *pos3, float *vel3, uint32_t *ids, uint32_t n, int by_id);
+
+/* Replaces: reading fields of CParticle::Physics back from the device
+ * (src/CGPUParticleSimulator.cpp:118, include/CParticle.h:22-43): cell ids, sort
+ * permutation, cell table, neighbour counts, density, pressure, acceleration. `bytes`
+ * is the capacity of out; too small is SPH_ERR_INVALID. */
+sph_status sph_debug(sph_ctx *ctx, int kind, void *out, size_t bytes);
+
+/* Replaces: CBaseParticleSimulator::getParticlesCount (include/CBaseParticleSimulator.h:52). */
+sph_status sph_count(const sph_ctx *ctx, uint32_t *n);
+
+/* Grid dimensions and cell count of the context (CGrid::xRes/yRes/zRes, include/CGrid.h:30-34). */
+sph_status sph_grid_dims(const sph_ctx *ctx, int *xyz, uint32_t *ncells);
+
+/* Replaces: getSelectedDevice (include/CBaseParticleSimulator.h:42,
+ * src/CGPUBaseParticleSimulator.cpp:17-20). Writes the device name. */
+sph_status sph_device_name(char *buf, size_t len);
+
+/* Waits for all work of the context. */
+sph_status sph_sync(sph_ctx *ctx);
+
+/* Replaces: CLWrapper::getEventDuration with PROFILING (src/CLWrapper.cpp:128-162).
+ * Accumulated device ms per stage (SPH_STAGE_COUNT floats) and launches counted, since
+ * creation or the last reset; needs SPH_FLAG_PROFILE. */
+sph_status sph_stage_times(sph_ctx *ctx, float *ms, uint32_t *launches, int reset);
+
+/* Replaces: CLException::what() (include/CLWrapper.h:18-22). Thread-local; never NULL. */
+const char *sph_last_error(void);
+
+/* Replaces: CBaseParticleSimulator::setupScene DAM_BREAK (src/CBaseParticleSimulator.cpp:38-65).
+ * Writes the reference's dam-break lattice for a cubic box into pos3 (id order) and
+ * returns the count in *n. With pos3 == NULL only *n is computed. */
+sph_status sph_scene_dam_break(float box, float *pos3, uint32_t cap, uint32_t *n);
+
+/* Benchmark lattice (no reference counterpart: the benchmark scene definition).
+ * nx*ny*nz sites at spacing h/2 from the box corner -box/2, nested y, x, z as the
+ * reference's setupScene loop, coordinates accumulated in float, each coordinate then
+ * jittered by (2u-1)*jitter_h*h with u from SplitMix64(seed) (24-bit uniforms drawn in
+ * id order, x then y then z). */
+sph_status sph_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float box[3],
+                             float jitter_h, uint64_t seed, float *pos3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
