@@ -26,3 +26,51 @@ def hip():
     import sphhip
     sphhip.load()
     return sphhip
+
+
+M64 = (1 << 64) - 1
+
+
+def splitmix64(seed, count):
+    """SplitMix64 (Steele, Lea, Flood 2014) stream as Python ints."""
+    s = seed & M64
+    out = []
+    for _ in range(count):
+        s = (s + 0x9E3779B97F4A7C15) & M64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def lattice_numpy(nx, ny, nz, box, jitter_h=0.01, seed=0):
+    """The benchmark lattice written independently of libsph_hip.so: sites at
+    spacing h/2 from the corner -box/2 (loop nest y, x, z; coordinates
+    accumulated in float), each coordinate jittered by (2u-1)*jitter_h*h."""
+    import numpy as np
+    h = np.float32(0.0457)
+    half = float(np.float32(h / np.float32(2)))
+
+    def coords(n):
+        c, v = [], np.float32(0)
+        for _ in range(n):
+            c.append(v)
+            v = np.float32(float(v) + half)
+        return np.array(c, np.float32)
+
+    b = np.broadcast_to(np.asarray(box, np.float32), (3,))
+    off = (-b / np.float32(2)).astype(np.float32)
+    cx, cy, cz = coords(nx), coords(ny), coords(nz)
+    Y, X, Z = np.meshgrid(cy, cx, cz, indexing="ij")
+    pos = np.stack([X.ravel() + off[0], Y.ravel() + off[1], Z.ravel() + off[2]], 1).astype(np.float32)
+    u = np.array([(v >> 40) for v in splitmix64(seed, pos.size)], np.float64) * 2.0 ** -24
+    amp = float(np.float32(jitter_h)) * float(h)
+    jit = ((2.0 * u - 1.0) * amp).astype(np.float32).reshape(-1, 3)
+    return (pos + jit).astype(np.float32)
+
+
+@pytest.fixture(scope="session")
+def scene_4k():
+    """The 4,096-particle dam-break lattice (16^3 at h/2, box 0.4, seed 0)."""
+    return lattice_numpy(16, 16, 16, 0.4, 0.01, 0)

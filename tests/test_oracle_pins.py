@@ -1,0 +1,361 @@
+"""Pins of the CPU oracle (the restated CCPUParticleSimulator) that do not
+involve the GPU. Each check is independent of the oracle's own code path:
+exact rational closed forms (tests/golden/closed_forms.json), brute force,
+textbook library routines (numpy stable argsort / searchsorted), hand-derived
+known answers and counts published in the reference's report. A plausible
+mistake in the oracle (wrong constant or exponent, float instead of double
+hash, missing neighbour cell, wrong sign, wrong wall normal, wrong integrator
+order, wrong bucket order) fails at least one of them.
+"""
+import json
+import os
+from decimal import Decimal
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "closed_forms.json")) as _f:
+    CF = json.load(_f)
+
+H = np.float32(0.0457)
+HSQ = np.float32(H * H)
+
+
+def rel(a, b):
+    return abs(float(a) - float(b)) / abs(float(b))
+
+
+# ---------------------------------------------------------------- constants
+
+def test_constants_match_exact_closed_forms(orc):
+    """poly6 = 315/(64 pi h^9), spiky = -45/(pi h^6), visc = 45/(pi h^6) at the
+    exact float32 h (src/CCPUParticleSimulator.cpp:11, 19, 26), to double rounding."""
+    c = orc.constants()
+    for key in ("poly6", "spiky", "visc"):
+        assert rel(c[key], Decimal(CF[key])) < 4e-16, key
+    assert np.float32(c["hsq"]).view(np.uint32) == int(CF["hsq_f_bits"], 16) == 0x3B08DF0C
+    assert c["h"] == float(np.float32(0.0457))
+
+
+def test_kernel_normalisation_and_derivatives(orc):
+    """The coefficients are the Mueller et al. normalisations: poly6 integrates
+    to 1 over the ball of radius h, and -45/(pi h^6)(h-r)^2 is dW/dr of the
+    normalised spiky kernel 15/(pi h^6)(h-r)^3; 45/(pi h^6)(h-r) is the Laplacian
+    of the normalised viscosity kernel (checked by quadrature / finite differences)."""
+    from scipy import integrate
+    c = orc.constants()
+    h = c["h"]
+    val, _ = integrate.quad(lambda r: 4 * np.pi * r * r * c["poly6"] * (h * h - r * r) ** 3, 0, h)
+    assert abs(val - 1) < 1e-6
+    spiky_w = 15.0 / (np.pi * h ** 6)
+    val, _ = integrate.quad(lambda r: 4 * np.pi * r * r * spiky_w * (h - r) ** 3, 0, h)
+    assert abs(val - 1) < 1e-6
+    r0, e = 0.6 * h, 1e-7 * h
+    dwdr = (spiky_w * (h - (r0 + e)) ** 3 - spiky_w * (h - (r0 - e)) ** 3) / (2 * e)
+    assert rel(dwdr, c["spiky"] * (h - r0) ** 2) < 1e-6
+    # viscosity kernel W = 15/(2 pi h^3)(-r^3/2h^3 + r^2/h^2 + h/2r - 1); Laplacian = 45/(pi h^6)(h - r)
+    wv = lambda r: 15 / (2 * np.pi * h ** 3) * (-r ** 3 / (2 * h ** 3) + r * r / (h * h) + h / (2 * r) - 1)
+    e = 1e-4 * h
+    lap = (wv(r0 + e) - 2 * wv(r0) + wv(r0 - e)) / e ** 2 + 2 / r0 * (wv(r0 + e) - wv(r0 - e)) / (2 * e)
+    assert rel(lap, c["visc"] * (h - r0)) < 1e-5
+
+
+def test_grid_resolution_and_dam_break_counts(orc):
+    """ceil(box/h) and the dam-break counts of the reference's report (doc.pdf:
+    1,620 particles at box 0.4 m, 16,000 at 0.9 m)."""
+    assert [orc.grid_res(b) for b in (0.4, 0.9, 1.6, 3.2)] == [9, 20, 36, 71]
+    assert orc.dam_break_count(0.4) == 1620
+    assert orc.dam_break_count(0.9) == 16000
+    for box, n in ((0.4, 1620), (0.9, 16000)):
+        s = orc.OracleSim(box)
+        assert s.setup_scene() == n
+        pos = s.state()["pos"]
+        # corner lattice: x in [-B/2, -B/4), y and z over the whole box, spacing h/2
+        assert pos[:, 0].min() == np.float32(-box / 2) and pos[:, 0].max() < -box / 4
+        steps = np.diff(np.unique(pos[:, 2]))
+        assert np.allclose(steps, float(H) / 2, rtol=1e-5)
+
+
+# ---------------------------------------------------------------- grid
+
+def test_hash_against_exact_arithmetic(orc):
+    """floor((p + B/2)/h) in exact rational arithmetic equals the oracle's cell
+    everywhere except within 1e-12 of a cell boundary (double rounding)."""
+    rng = np.random.default_rng(1)
+    box = np.float32(0.4)
+    pos = ((rng.random((4000, 3)) - 0.5) * 0.39).astype(np.float32)
+    cells = orc.hash_positions(pos, box, 9)
+    hq = Fraction(float(H))
+    half = Fraction(float(box)) / 2
+    for p, c in zip(pos[:500], cells[:500]):
+        xyz = []
+        for a in range(3):
+            q = (Fraction(float(p[a])) + half) / hq
+            xyz.append(min(max(int(np.floor(float(q))) if q.denominator != 1 else int(q), 0), 8))
+        assert c == xyz[0] + 9 * xyz[1] + 81 * xyz[2]
+
+
+def test_hash_uses_double_not_float_division(orc):
+    """Within a few float ulps of a cell boundary a float32 quotient rounds into
+    the wrong cell where the reference's double arithmetic (and exact rational
+    arithmetic) does not: the oracle must follow the exact cell there."""
+    box = np.float32(0.4)
+    found = 0
+    for c in range(1, 9):
+        b = np.float32(float(Fraction(-float(box)) / 2 + c * Fraction(float(H))))
+        cand = [b]
+        for direction in (np.float32(1), np.float32(-1)):
+            x = b
+            for _ in range(60):
+                x = np.nextafter(x, direction)
+                cand.append(x)
+        for x in cand:
+            q = (Fraction(float(x)) + Fraction(float(box)) / 2) / Fraction(float(H))
+            if abs(float(q) - round(float(q))) < 1e-12:
+                continue
+            exact = int(np.floor(float(q)))
+            flt = int(np.floor((np.float32(x) + box / np.float32(2)) / H))
+            if flt != exact:
+                got = int(orc.hash_positions(np.array([[x, 0, 0]], np.float32), box, 9)[0]) % 9
+                assert got == exact, (c, x)
+                found += 1
+    assert found > 0
+
+
+def test_hash_clamps_and_non_finite_like_x86(orc):
+    """Outside the box the cell clamps to the edge; NaN, infinities and values
+    beyond int range hit the x86 cvttsd2si 'integer indefinite' 0x80000000
+    (Intel SDM, CVTTSD2SI) and then clamp to 0, as the reference binary does."""
+    pos = np.array([[-0.3, 0, 0], [0.3, 0, 0], [np.nan, 0, 0], [np.inf, 0, 0], [-np.inf, 0, 0],
+                    [1e30, 0, 0], [0, 0, 0]], np.float32)
+    cells = orc.hash_positions(pos, 0.4, 9) % 9
+    assert cells.tolist() == [0, 8, 0, 0, 0, 0, 4]
+
+
+def test_stable_perm_and_cell_start_are_textbook(orc):
+    rng = np.random.default_rng(2)
+    keys = rng.integers(0, 300, 5000).astype(np.uint32)
+    perm = orc.stable_perm(keys, 300)
+    np.testing.assert_array_equal(perm, np.argsort(keys, kind="stable"))
+    sk = keys[perm]
+    np.testing.assert_array_equal(orc.cell_start(sk, 300), np.searchsorted(sk, np.arange(301), side="left"))
+
+
+def test_bucket_order_swap_remove_known_answer(orc):
+    """src/CCPUParticleSimulator.cpp:72-84 by hand: particles 0..3 start in cell
+    (0,0,0); 0, 2 and 3 move to cell A, 1 stays. Visiting cell 0: 0 moves (3 is
+    swapped in), 3 moves (2 swapped in), 2 moves, 1 stays -> A = [0, 3, 2], cell0 = [1]."""
+    s = orc.OracleSim(0.4, 8)
+    lo = np.float32(-0.19)
+    a = np.float32(0.1)
+    for p in ([a, a, a], [lo, lo, lo], [a, a, a], [a, a, a]):
+        s.add_particle(*p)
+    s.update_grid()
+    start, ids = s.buckets()
+    cell_a = int(orc.hash_positions(np.array([[a, a, a]], np.float32), 0.4, 9)[0])
+    assert ids[start[0]:start[1]].tolist() == [1]
+    assert ids[start[cell_a]:start[cell_a + 1]].tolist() == [0, 3, 2]
+
+
+def test_buckets_partition_particles(orc, scene_4k):
+    for mode in (orc.ORDER_REFERENCE, orc.ORDER_CANONICAL):
+        s = orc.OracleSim(0.4, len(scene_4k))
+        s.add_particles(scene_4k)
+        s.set_order_mode(mode)
+        s.step(3)
+        s.update_grid()
+        start, ids = s.buckets()
+        cells = s.cells()
+        assert np.array_equal(np.sort(ids), np.arange(len(scene_4k)))
+        owner = np.repeat(np.arange(len(start) - 1), np.diff(start))
+        np.testing.assert_array_equal(cells[ids], owner)
+
+
+def test_canonical_order_is_stable_sort_of_previous_order(orc, scene_4k):
+    s = orc.OracleSim(0.4, len(scene_4k))
+    s.add_particles(scene_4k)
+    s.set_order_mode(orc.ORDER_CANONICAL)
+    s.update_grid()
+    _, order1 = s.buckets()
+    np.testing.assert_array_equal(order1, np.argsort(s.cells(), kind="stable"))
+    s.update_density_pressure(); s.update_forces(); s.integrate()
+    s.update_grid()
+    _, order2 = s.buckets()
+    np.testing.assert_array_equal(order2, order1[np.argsort(s.cells()[order1], kind="stable")])
+
+
+# ---------------------------------------------------------------- density
+
+def _brute_pairs(pos):
+    d = pos[:, None, :] - pos[None, :, :]
+    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    return d, r2.astype(np.float32)
+
+
+def test_neighbour_counts_equal_all_pairs(orc):
+    """27-cell candidate sets find every pair with r^2 <= fl(h^2) (float32 sums
+    in QVector3D::lengthSquared order), i.e. the all-pairs semantics."""
+    rng = np.random.default_rng(3)
+    pos = ((rng.random((1500, 3)) - 0.5) * np.float32(0.38)).astype(np.float32)
+    pos[:200] = (pos[:200] * np.float32(0.2)).astype(np.float32)  # a dense core
+    _, r2 = _brute_pairs(pos)
+    expect = (r2 <= HSQ).sum(1)
+    for mode in (orc.ORDER_REFERENCE, orc.ORDER_CANONICAL):
+        s = orc.OracleSim(0.4, len(pos))
+        s.add_particles(pos)
+        s.set_order_mode(mode)
+        s.set_diagnostics(True)
+        s.update_grid()
+        s.update_density_pressure()
+        np.testing.assert_array_equal(s.neighbour_counts(), expect)
+
+
+def test_density_matches_high_precision_brute_force(orc):
+    rng = np.random.default_rng(4)
+    pos = ((rng.random((1200, 3)) - 0.5) * np.float32(0.3)).astype(np.float32)
+    _, r2 = _brute_pairs(pos)
+    c = orc.constants()
+    w = np.where(r2 <= HSQ, (c["hsq"] - r2.astype(np.float64)) ** 3, 0.0)
+    rho = float(np.float32(0.02)) * c["poly6"] * w.sum(1)
+    s = orc.OracleSim(0.4, len(pos))
+    s.add_particles(pos)
+    s.update_grid()
+    s.update_density_pressure()
+    st = s.state()
+    assert np.max(np.abs(st["density"] - rho) / rho) < 2e-6
+    p = 3.0 * (rho - float(np.float32(998.29)))
+    assert np.max(np.abs(st["pressure"] - p)) < 5e-3
+
+
+def test_isolated_and_lattice_interior_density(orc):
+    s = orc.OracleSim(0.4, 200)
+    s.add_particle(0.1, 0.1, 0.1)
+    sp = np.float32(H / np.float32(2))
+    offs = [(i, j, k) for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)]
+    for i, j, k in offs:
+        s.add_particle(np.float32(-0.05) + np.float32(i) * sp, np.float32(-0.05) + np.float32(j) * sp,
+                       np.float32(-0.05) + np.float32(k) * sp)
+    s.set_diagnostics(True)
+    s.update_grid()
+    s.update_density_pressure()
+    st = s.state()
+    assert rel(st["density"][0], Decimal(CF["isolated_density"])) < 1e-6
+    centre = 1 + offs.index((0, 0, 0))
+    assert s.neighbour_counts()[centre] == CF["lattice_interior_count"] == 33
+    assert rel(st["density"][centre], Decimal(CF["lattice_interior_density"])) < 2e-6
+
+
+# ---------------------------------------------------------------- forces
+
+@pytest.mark.parametrize("k", range(4))
+def test_pair_force_closed_form(orc, k):
+    """Two particles (no walls in reach): density, pressure and the pressure +
+    viscosity acceleration against the exact closed form."""
+    pr = CF["pairs"][k]
+    s = orc.OracleSim(0.4, 4)
+    s.set_gravity((0, 0, 0))
+    s.add_particle(0, 0, 0)
+    s.add_particle(pr["r"], 0, 0, pr["u"], 0, 0)
+    s.update_grid()
+    s.update_density_pressure()
+    s.update_forces()
+    st = s.state()
+    assert rel(st["density"][0], Decimal(pr["density"])) < 1e-6
+    assert abs(float(st["pressure"][0]) - float(Decimal(pr["pressure"]))) < 2e-3
+    assert rel(st["acc"][0, 0], Decimal(pr["ax_sph"])) < 1e-5
+    assert st["acc"][0, 1] == 0 and st["acc"][0, 2] == 0
+    # equal densities: the partner's pressure and viscosity terms are opposite
+    assert rel(st["acc"][1, 0], -float(Decimal(pr["ax_sph"]))) < 1e-5
+
+
+def test_pressure_forces_cancel(orc, scene_4k):
+    """With zero velocities the viscosity term vanishes and the pressure forces
+    are pairwise antisymmetric: sum_i (a_i - g - wall_i) = 0 up to rounding of
+    the sum of their magnitudes."""
+    s = orc.OracleSim(0.4, len(scene_4k))
+    s.add_particles(scene_4k)
+    s.set_diagnostics(True)
+    s.update_grid()
+    s.update_density_pressure()
+    s.update_forces()
+    st = s.state()
+    wall = np.array([orc.wall_accel(p, np.zeros(3, np.float32), 0.4) for p in st["pos"]])
+    g = np.array([0, -9.80665, 0])
+    f = (st["acc"].astype(np.float64) - g - wall)
+    scale = s.force_scale().astype(np.float64).sum(0)
+    assert np.all(np.abs(f.sum(0)) < 1e-5 * scale)
+    assert np.abs(f).sum() > 1e3  # the cancelled sum is not trivially zero
+
+
+def test_wall_response_closed_form(orc):
+    """inverseBoundingBoxBounce: within 0.01 of a wall the acceleration is
+    1e4*(0.01 - depth) along the inward normal plus -0.9*(v . n')*n'."""
+    box = 0.4
+    cases = [
+        ([-0.195, 0.0, 0.0], [0.3, -0.2, 0.1], [1e4 * (0.01 - 0.005) - 0.9 * 0.3, 0, 0]),
+        ([0.0, 0.197, 0.0], [0.0, 0.5, 0.0], [0, -1e4 * (0.01 - 0.003) - 0.9 * 0.5, 0]),
+        ([0.0, 0.0, -0.2], [0.0, 0.0, -1.0], [0, 0, 1e4 * 0.01 - 0.9 * -1.0]),
+        ([0.192, -0.199, 0.0], [1.0, 1.0, 0.0], [-1e4 * (0.01 - 0.008) - 0.9 * 1.0, 1e4 * (0.01 - 0.001) - 0.9 * 1.0, 0]),
+        ([0.0, 0.0, 0.0], [1.0, 2.0, 3.0], [0, 0, 0]),
+    ]
+    for p, v, expect in cases:
+        got = orc.wall_accel(np.array(p, np.float32), np.array(v, np.float32), box)
+        np.testing.assert_allclose(got, expect, rtol=2e-4, atol=2e-4)
+
+
+def test_free_fall_symplectic_euler(orc):
+    """An isolated particle far from the walls: v_n = n g dt and
+    y_n = y_0 + g dt^2 n (n+1)/2 (x' = x + v dt + a dt^2, v' = (x'-x)/dt), from
+    y_0 = 0 so that float cancellation in (x'-x) stays below the tolerance."""
+    s = orc.OracleSim(3.2, 1)
+    s.add_particle(0.0, 0.0, 0.0)
+    done = 0
+    for ff in CF["free_fall"]:
+        s.step(ff["n"] - done)
+        done = ff["n"]
+        st = s.state()
+        assert rel(st["pos"][0, 1], Decimal(ff["dy"])) < 1e-5
+        assert rel(st["vel"][0, 1], Decimal(ff["vy"])) < 1e-5
+
+
+def test_canonical_and_reference_order_agree_to_rounding(orc, scene_4k):
+    """Same terms in another order: identical neighbour counts, densities within
+    1e-6, and trajectories within 1e-3*h over 10 steps (before chaos grows)."""
+    res = {}
+    for mode in (orc.ORDER_REFERENCE, orc.ORDER_CANONICAL):
+        s = orc.OracleSim(0.4, len(scene_4k))
+        s.add_particles(scene_4k)
+        s.set_order_mode(mode)
+        s.set_diagnostics(True)
+        s.update_grid()
+        s.update_density_pressure()
+        res[mode] = (s.neighbour_counts(), s.state()["density"])
+        s.update_forces(); s.integrate()
+        s.step(9)
+        res[mode] += (s.state()["pos"],)
+    a, b = res[orc.ORDER_REFERENCE], res[orc.ORDER_CANONICAL]
+    np.testing.assert_array_equal(a[0], b[0])
+    assert np.max(np.abs(a[1] - b[1]) / a[1]) < 1e-6
+    assert np.max(np.abs(a[2].astype(np.float64) - b[2])) < 1e-3 * float(H)
+
+
+def test_fountain_generation_known_answer(orc):
+    """generateParticles (src/CBaseParticleSimulator.cpp:187-210): 7 particles per
+    step at the floor with v_y = 3.2*B; a step adds nothing once count >= max - 7."""
+    s = orc.OracleSim(0.1, scenario=orc.FOUNTAIN)
+    s.setup_scene()
+    assert s.max_count == 50
+    s.step(1)
+    st = s.state()
+    assert s.count == 7
+    hp = np.float32(H / np.float32(2))
+    q = np.float32(H / np.float32(4))
+    y0 = np.float32(-0.05)
+    expect0 = np.array([[0, y0, 0], [-hp, y0, 0], [hp, y0, 0], [-q, y0, -hp], [q, y0, -hp], [-q, y0, hp],
+                        [q, y0, hp]], np.float32)
+    # after one step the particles moved by v*dt + a*dt^2 from the emitter points
+    assert np.all(np.abs(st["pos"][:, [0, 2]] - expect0[:, [0, 2]]) < 0.01)
+    s.step(20)
+    assert s.count == 49  # 42 < 50 - 7 adds a last batch of 7; 49 >= 43 stops
