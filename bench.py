@@ -71,7 +71,6 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dam_break_1m")
-    ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -103,12 +102,12 @@ def main():
     value = n * args.steps / wall
 
     # Per-stage device times from HIP events on the library's stream, in a
-    # separate profiled context over the same workload (after the same warmup).
+    # separate profiled context over the same steps as the timed window.
     prof = sphhip.HIPGridParticleSimulator(box, capacity=n, flags=_lib.FLAG_PROFILE)
     prof.upload(pos)
     prof.advance(args.warmup) if args.warmup else None
     prof.stage_times(reset=True)
-    prof.advance(args.profile_steps)
+    prof.advance(args.steps)
     st = prof.stage_times()
     stages = {k: (ms / cnt if cnt else 0.0) for k, (ms, cnt) in st.items()}
     dominant = max(("density", "forces"), key=lambda k: stages[k])

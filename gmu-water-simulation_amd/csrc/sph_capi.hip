@@ -36,6 +36,7 @@ sph_status fail(sph_status code, const std::string &msg)
     } while (0)
 
 constexpr int kProfileSlots = 256;  // event pairs buffered before a forced drain
+constexpr uint32_t kDefaultMaxNeighbours = 192;
 
 }  // namespace
 
@@ -59,12 +60,18 @@ struct sph_ctx {
     uint32_t *key_h = nullptr, *key_t = nullptr, *key_s = nullptr;
     uint32_t *val_t = nullptr, *val_s = nullptr;
     uint32_t *hist = nullptr;
+    uint32_t *partials = nullptr;
     uint32_t *cstart = nullptr;
 
     float *rho = nullptr;
     double *prr = nullptr;
+    double *rinv = nullptr;
+    float4 *rec = nullptr;
     float *acc[3] = {};
     uint32_t *ncount = nullptr;
+    uint32_t *nlist = nullptr;   // neighbour lists, max_neighbours x capacity
+    uint32_t *nlist_cnt = nullptr;
+    uint32_t kmax = 0;
 
     bool keys_valid = false;   // key_h holds the cells of the current positions
     bool grid_valid = false;   // sorted order / cell table match the current positions
@@ -81,6 +88,27 @@ struct sph_ctx {
     hipEvent_t span[2] = {};
 
     std::vector<void *> allocations;
+
+    sph::DensityFields fields(bool diag) const
+    {
+        sph::DensityFields f;
+        f.rho = rho;
+        f.prr = prr;
+        f.rinv = rinv;
+        f.ncount = diag ? ncount : nullptr;
+        f.rec = rec;
+        return f;
+    }
+
+    sph::NeighbourList nlists() const
+    {
+        sph::NeighbourList nl;
+        nl.idx = nlist;
+        nl.cnt = nlist_cnt;
+        nl.kmax = kmax;
+        nl.stride = cap;
+        return nl;
+    }
 
     sph::ParticleSoA soa(int which, int idw) const
     {
@@ -229,7 +257,7 @@ sph_status run_grid(sph_ctx *c)
             const bool to_s = ((passes - 1 - p) % 2) == 0;
             uint32_t *kout = to_s ? c->key_s : c->key_t;
             uint32_t *vout = to_s ? c->val_s : c->val_t;
-            sph::launch_radix_pass(kin, vin, kout, vout, n, p * sph::kRadixBits, c->hist, c->stream);
+            sph::launch_radix_pass(kin, vin, kout, vout, n, p * sph::kRadixBits, c->hist, c->partials, c->stream);
             kin = kout;
             vin = vout;
         }
@@ -259,8 +287,8 @@ sph_status run_density(sph_ctx *c)
     StageTimer t(c, SPH_STAGE_DENSITY);
     if (t.st) return t.st;
     const bool diag = (c->prm.flags & SPH_FLAG_DIAGNOSTICS) != 0;
-    sph::launch_density(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->rho, c->prr, diag ? c->ncount : nullptr,
-                        c->scene, c->k, c->stream);
+    sph::launch_density(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->fields(diag), c->nlists(), c->scene,
+                        c->k, c->stream);
     return launch_error("density launch");
 }
 
@@ -271,7 +299,8 @@ sph_status run_fused_tail(sph_ctx *c)
     const bool diag = (c->prm.flags & SPH_FLAG_DIAGNOSTICS) != 0;
     sph::ParticleSoA in = c->soa(c->cur, c->id_cur);
     sph::ParticleSoA out = c->soa(1 - c->cur, c->id_cur);
-    sph::launch_forces_fused(in, c->key_s, c->cstart, c->rho, c->prr, out, c->key_h, diag ? c->acc[0] : nullptr,
+    sph::launch_forces_fused(in, c->key_s, c->cstart, c->fields(diag), c->nlists(), out, c->key_h,
+                             diag ? c->acc[0] : nullptr,
                              diag ? c->acc[1] : nullptr, diag ? c->acc[2] : nullptr, c->scene, c->k, c->stream);
     sph_status st = launch_error("fused forces launch");
     if (st) return st;
@@ -307,8 +336,8 @@ sph_status run_phase(sph_ctx *c, int phase)
     case SPH_PHASE_FORCES: {
         StageTimer t(c, SPH_STAGE_FORCES);
         if (t.st) return t.st;
-        sph::launch_forces(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->rho, c->prr, c->acc[0], c->acc[1],
-                           c->acc[2], c->scene, c->k, c->stream);
+        sph::launch_forces(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->fields(false), c->nlists(), c->acc[0],
+                           c->acc[1], c->acc[2], c->scene, c->k, c->stream);
         st = launch_error("forces launch");
         break;
     }
@@ -388,6 +417,7 @@ void sph_default_params(sph_params *p, float box, uint32_t capacity)
     p->dt = sph::kDt;
     p->capacity = capacity;
     p->flags = 0;
+    p->max_neighbours = 0;
 }
 
 int sph_grid_res(float box)
@@ -405,6 +435,8 @@ sph_status sph_create(const sph_params *params, sph_ctx **out)
             return fail(SPH_ERR_INVALID, "sph_create: box extents must be positive and finite");
     if (params->capacity == 0) return fail(SPH_ERR_INVALID, "sph_create: capacity must be > 0");
     if (!(params->dt > 0.0f)) return fail(SPH_ERR_INVALID, "sph_create: dt must be > 0");
+    if (params->max_neighbours > 4096 && params->max_neighbours != SPH_NEIGHBOUR_LISTS_OFF)
+        return fail(SPH_ERR_INVALID, "sph_create: max_neighbours above 4096");
     const int rx = sph_grid_res(params->box[0]), ry = sph_grid_res(params->box[1]), rz = sph_grid_res(params->box[2]);
     const uint64_t ncells = (uint64_t) rx * (uint64_t) ry * (uint64_t) rz;
     if (ncells == 0 || ncells >= (1ull << 31))
@@ -424,6 +456,8 @@ sph_status sph_create(const sph_params *params, sph_ctx **out)
     c->prm = *params;
     c->device = dev;
     c->cap = params->capacity;
+    c->kmax = params->max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0u
+              : params->max_neighbours ? params->max_neighbours : kDefaultMaxNeighbours;
     c->ncells = (uint32_t) ncells;
     c->k = sph::make_consts();
     c->scene.rx = rx;
@@ -459,12 +493,18 @@ sph_status sph_create(const sph_params *params, sph_ctx **out)
     if (!st) st = dalloc(c, &c->key_s, P);
     if (!st) st = dalloc(c, &c->val_t, P);
     if (!st) st = dalloc(c, &c->val_s, P);
-    if (!st) st = dalloc(c, &c->hist, (size_t) sph::kRadixBuckets * sph::sort_blocks(c->cap));
+    const size_t hist_n = (size_t) sph::kRadixBuckets * sph::sort_blocks(c->cap);
+    if (!st) st = dalloc(c, &c->hist, hist_n);
+    if (!st) st = dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n));
     if (!st) st = dalloc(c, &c->cstart, (size_t) c->ncells + 1);
     if (!st) st = dalloc(c, &c->rho, P);
     if (!st) st = dalloc(c, &c->prr, P);
+    if (!st) st = dalloc(c, &c->rinv, P);
+    if (!st) st = dalloc(c, &c->rec, 4 * P);
     for (int a = 0; a < 3 && !st; ++a) st = dalloc(c, &c->acc[a], P);
     if (!st) st = dalloc(c, &c->ncount, P);
+    if (!st) st = dalloc(c, &c->nlist_cnt, P);
+    if (!st) st = dalloc(c, &c->nlist, (size_t) c->kmax * P);
     if (!st && (params->flags & SPH_FLAG_PROFILE)) {
         for (auto &pair : c->ev)
             for (auto &ev : pair)

@@ -15,21 +15,48 @@ struct ParticleSoA {
     uint32_t *id;
 };
 
+// Per-particle results of the density pass that the force pass reads.
+struct DensityFields {
+    float *rho;       // density
+    double *prr;      // p / rho^2 in double (src/CCPUParticleSimulator.cpp:181)
+    double *rinv;     // 1 / rho in double (exact-division helper of the force pass)
+    uint32_t *ncount; // accepted neighbours incl. self (diagnostics; may be null)
+    // Everything the force pass reads about a neighbour, one 64-byte record per
+    // slot so that a pair costs three 16-byte loads from one cache line:
+    // {x, y, z, vx}, {vy, vz, -, -}, {p/rho^2, 1/rho} (doubles), 16 bytes padding.
+    float4 *rec;
+};
+
+// Accepted neighbours of every particle except itself, in canonical order,
+// stored neighbour-major: idx[k * stride + i] is the k-th neighbour of slot i.
+// cnt[i] is the true count; particles with cnt > kmax have no usable list and
+// the force pass re-walks their cells.
+struct NeighbourList {
+    uint32_t *idx;
+    uint32_t *cnt;
+    uint32_t kmax;
+    uint32_t stride;
+};
+
 constexpr int kRadixBits = 8;
 constexpr int kRadixBuckets = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanTile = 4096;  // elements per 1024-thread scan block
 
 inline uint32_t sort_blocks(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+inline uint32_t scan_blocks(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 
 // Cell key of every live particle (hash of src/CCPUParticleSimulator.cpp:46-69).
 void launch_hash(const ParticleSoA &p, uint32_t *key, const Scene &s, double h, hipStream_t st);
 
-// One stable LSD radix pass over `bits` bits starting at `shift`. vals_in == nullptr
-// means the identity (val = slot). hist must hold kRadixBuckets * sort_blocks(n).
+// One stable LSD radix pass over 8 bits starting at `shift`. vals_in == nullptr
+// means the identity (val = slot). hist holds kRadixBuckets * sort_blocks(n)
+// entries, partials scan_blocks(kRadixBuckets * sort_blocks(n)) entries.
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
-                       uint32_t *vals_out, uint32_t n, int shift, uint32_t *hist, hipStream_t st);
+                       uint32_t *vals_out, uint32_t n, int shift, uint32_t *hist, uint32_t *partials,
+                       hipStream_t st);
 
 // start[c] = first slot whose sorted key is >= c, c in [0, ncells].
 void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells, uint32_t *start,
@@ -39,13 +66,13 @@ void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells,
 void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32_t *perm, uint32_t n,
                     hipStream_t st);
 
-// Density and p/rho^2 (double) of every particle; ncount may be null.
-void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, float *rho,
-                    double *prr, uint32_t *ncount, const Scene &s, const Consts &k, hipStream_t st);
+// Density pass: density, p/rho^2, 1/rho and the neighbour lists.
+void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
+                    const NeighbourList &nl, const Scene &s, const Consts &k, hipStream_t st);
 
 // Pressure + viscosity + gravity acceleration into acc (the updateForces phase).
-void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const float *rho,
-                   const double *prr, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
+void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
+                   const NeighbourList &nl, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
                    hipStream_t st);
 
 // acc += wall response (the updateCollisions phase).
@@ -59,7 +86,8 @@ void launch_integrate(const ParticleSoA &p, const float *ax, const float *ay, co
 
 // Fused forces + walls + integrate + next hash. acc pointers may be null (then not stored).
 void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
-                         const float *rho, const double *prr, const ParticleSoA &out, uint32_t *key_next,
-                         float *ax, float *ay, float *az, const Scene &s, const Consts &k, hipStream_t st);
+                         const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out,
+                         uint32_t *key_next, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
+                         hipStream_t st);
 
 }  // namespace sph

@@ -133,49 +133,86 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__
     for (int d = t; d < kRadixBuckets; d += kSortThreads) hist[(uint32_t) d * nblocks + blockIdx.x] = h[d];
 }
 
-// In-place exclusive scan of m uint32 by one 1024-thread workgroup, 4096 per tile:
-// wave64 shuffle scans, then a scan of the 16 wave totals.
+// Exclusive scan of a 4096-element tile held 4 per thread by a 1024-thread block:
+// wave64 shuffle scans, then a scan of the 16 wave totals. Returns the thread's
+// exclusive prefix within the tile and the tile total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t s, uint32_t *wsum, uint32_t &total)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t x = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        const uint32_t ws = lane < 16 ? wsum[lane] : 0u;
+        uint32_t xs = ws;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const uint32_t y = __shfl_up(xs, d, 64);
+            if (lane >= d) xs += y;
+        }
+        if (lane < 16) wsum[lane] = xs - ws;
+        if (lane == 15) wsum[16] = xs;
+    }
+    __syncthreads();
+    total = wsum[16];
+    const uint32_t r = wsum[w] + (x - s);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint32_t load4(const uint32_t *__restrict__ data, uint32_t idx, uint32_t m, uint32_t v[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = (idx + q < m) ? data[idx + q] : 0u;
+    return v[0] + v[1] + v[2] + v[3];
+}
+
+// Single-block in-place exclusive scan of m values (used on the block totals).
 __global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t *__restrict__ data, uint32_t m)
 {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry_s;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    __shared__ uint32_t wsum[17];
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < m; base += 4096u) {
-        const uint32_t idx = base + (uint32_t) t * 4u;
-        uint32_t v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = (idx + q < m) ? data[idx + q] : 0u;
-        const uint32_t s = v[0] + v[1] + v[2] + v[3];
-        uint32_t x = s;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        if (lane == 63) wsum[w] = x;
-        __syncthreads();
-        if (w == 0) {
-            const uint32_t ws = lane < 16 ? wsum[lane] : 0u;
-            uint32_t xs = ws;
-#pragma unroll
-            for (int d = 1; d < 16; d <<= 1) {
-                const uint32_t y = __shfl_up(xs, d, 64);
-                if (lane >= d) xs += y;
-            }
-            if (lane < 16) wsum[lane] = xs - ws;
-        }
-        __syncthreads();
-        uint32_t run = carry + wsum[w] + (x - s);
+    for (uint32_t base = 0; base < m; base += kScanTile) {
+        const uint32_t idx = base + (uint32_t) threadIdx.x * 4u;
+        uint32_t v[4], total;
+        uint32_t run = carry + block_exclusive_scan(load4(data, idx, m, v), wsum, total);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (idx + q < m) data[idx + q] = run;
             run += v[q];
         }
-        if (t == 1023) carry_s = run;
-        __syncthreads();
-        carry = carry_s;
-        __syncthreads();
+        carry += total;
+    }
+}
+
+// Multi-block scan, step 1: the total of every 4096-element tile.
+__global__ __launch_bounds__(1024) void k_scan_reduce(const uint32_t *__restrict__ data, uint32_t m,
+                                                      uint32_t *__restrict__ partials)
+{
+    __shared__ uint32_t wsum[17];
+    const uint32_t idx = blockIdx.x * (uint32_t) kScanTile + (uint32_t) threadIdx.x * 4u;
+    uint32_t v[4], total;
+    block_exclusive_scan(load4(data, idx, m, v), wsum, total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Multi-block scan, step 3: exclusive scan of every tile plus its scanned total.
+__global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data, uint32_t m,
+                                                     const uint32_t *__restrict__ partials)
+{
+    __shared__ uint32_t wsum[17];
+    const uint32_t idx = blockIdx.x * (uint32_t) kScanTile + (uint32_t) threadIdx.x * 4u;
+    uint32_t v[4], total;
+    uint32_t run = partials[blockIdx.x] + block_exclusive_scan(load4(data, idx, m, v), wsum, total);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (idx + q < m) data[idx + q] = run;
+        run += v[q];
     }
 }
 
@@ -263,99 +300,337 @@ __global__ void k_reorder(ParticleSoA src, ParticleSoA dst, const uint32_t *__re
 // the contiguous run of cells max(x-1,0)..min(x+1,X-1) in slot order. Rows off
 // the grid are skipped, as the reference's continue/break bounds tests skip
 // them (src/CCPUParticleSimulator.cpp:108-118).
+//
+// A wave walks the 9 rows together; each lane tests its own candidates and
+// appends the accepted ones, in order, to a private list in LDS. When any lane's
+// list is full every lane drains its list through the pair body, so the costly
+// pair arithmetic runs on accepted pairs only (about 1 in 6 candidates) and the
+// summation order of every particle is unchanged.
 
 constexpr int kNeighbourThreads = 256;
+constexpr int kListCap = 16;
 
-// updateDensityPressure, src/CCPUParticleSimulator.cpp:93-141.
-__global__ __launch_bounds__(kNeighbourThreads) void k_density(ParticleSoA p, const uint32_t *__restrict__ cell,
-                                                               const uint32_t *__restrict__ start,
-                                                               float *__restrict__ rho, double *__restrict__ prr,
-                                                               uint32_t *__restrict__ ncount, Scene s, Consts k)
+constexpr int kUnroll = 4;
+constexpr uint32_t kBufWord3 = 0x00020000;  // descriptor word 3 of a gfx9 raw buffer
+
+// Raw buffer views of the per-particle arrays: loads take 32-bit byte offsets
+// (constant parts fold into the instruction) instead of 64-bit addresses, and
+// reads past the end return 0 instead of faulting.
+struct BufView {
+    __amdgpu_buffer_rsrc_t px, py, pz, rho, rec;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *base, uint32_t bytes)
 {
+    return __builtin_amdgcn_make_buffer_rsrc((void *) base, (short) 0, (int) bytes, (int) kBufWord3);
+}
+
+__device__ __forceinline__ BufView buf_view(const ParticleSoA &p, const DensityFields &f, uint32_t slots)
+{
+    BufView v;
+    v.px = rsrc(p.px, slots * 4u);
+    v.py = rsrc(p.py, slots * 4u);
+    v.pz = rsrc(p.pz, slots * 4u);
+    v.rho = rsrc(f.rho, slots * 4u);
+    v.rec = rsrc(f.rec, slots * 64u);
+    return v;
+}
+
+__device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    return __uint_as_float((uint32_t) __builtin_amdgcn_raw_buffer_load_b32(r, (int) byte_off, 0, 0));
+}
+
+__device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int) byte_off, 0, 0));
+}
+
+// Squared lower bounds on the distance from a particle to any particle of the
+// neighbour cells at offset -1 / +1 along one axis: the particle's distance to
+// its own cell's lower / upper face (0 when it sits outside its clamped cell).
+struct FaceBounds {
+    double lo2, hi2;
+};
+
+__device__ __forceinline__ FaceBounds face_bounds(float p, float box, int c, double h)
+{
+    const double q = (double) p + (double) box / 2.0;
+    const double dlo = q - (double) c * h, dhi = (double) (c + 1) * h - q;
+    FaceBounds b;
+    b.lo2 = dlo > 0.0 ? dlo * dlo : 0.0;
+    b.hi2 = dhi > 0.0 ? dhi * dhi : 0.0;
+    return b;
+}
+
+// Walks the particle's candidates in canonical order: rows (z-1..z+1 outer,
+// y-1..y+1), each a contiguous run of the cells x-1..x+1. Rows and end cells of
+// a run whose distance lower bound exceeds h (with a 1e-5 relative margin, far
+// above the float rounding of r^2) are left out: every candidate they hold would
+// fail the r^2 <= h^2 test, so the accepted neighbours and their order are
+// unchanged. test(j, valid, xj, yj, zj) sees kUnroll candidates per trip in slot
+// order and appends without branching (writes slot cnt, then adds the accept
+// bit), so a list keeps kUnroll slots of headroom; when any lane's list is that
+// full, flush() drains every lane's list.
+template <typename Test, typename Flush>
+__device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k, const uint32_t *__restrict__ start,
+                                                const BufView &bv, int x, int y, int z, float xi, float yi,
+                                                float zi, int &cnt, Test test, Flush flush)
+{
+    const FaceBounds bx = face_bounds(xi, s.bx, x, k.h);
+    const FaceBounds by = face_bounds(yi, s.by, y, k.h);
+    const FaceBounds bz = face_bounds(zi, s.bz, z, k.h);
+    const double lim = k.hsq * (1.0 + 1e-5);
+    for (int r = 0; r < 9; ++r) {
+        const int oz = r / 3 - 1, oy = r % 3 - 1;
+        const int zz = z + oz, yy = y + oy;
+        const double lyz = (oy < 0 ? by.lo2 : oy > 0 ? by.hi2 : 0.0) + (oz < 0 ? bz.lo2 : oz > 0 ? bz.hi2 : 0.0);
+        uint32_t j = 0, jend = 0;
+        if (zz >= 0 && zz < s.rz && yy >= 0 && yy < s.ry && lyz <= lim) {
+            const uint32_t row = s.coff + (uint32_t) (yy * s.rx + zz * s.rx * s.ry);
+            const int x0 = (x > 0 && lyz + bx.lo2 <= lim) ? x - 1 : x;
+            const int x1 = (x + 1 < s.rx && lyz + bx.hi2 <= lim) ? x + 1 : x;
+            j = start[row + x0];
+            jend = start[row + x1 + 1];
+        }
+        while (__any(j < jend)) {
+            if (j < jend) {
+                const uint32_t off = j * 4u, rem = jend - j;
+                float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    cx[u] = ldf(bv.px, off + 4u * u);
+                    cy[u] = ldf(bv.py, off + 4u * u);
+                    cz[u] = ldf(bv.pz, off + 4u * u);
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) test(j + u, (uint32_t) u < rem, cx[u], cy[u], cz[u]);
+                j += kUnroll;
+            }
+            if (__any(cnt > kListCap - kUnroll)) {
+                flush();
+                cnt = 0;
+            }
+        }
+    }
+    flush();
+    cnt = 0;
+}
+
+// Float quotients a_c / b correctly rounded, from double products a_c * rb with
+// rb ~ 1/b to within 2^-52: for float a, b with a normal-range quotient the exact
+// a/b is never within 2^-49 (relative) of a float rounding midpoint (a = b*m for
+// a 25-bit midpoint m is impossible, and otherwise |a - b*m| is at least one unit
+// of b*m's last bit), so the rounded product is the IEEE float quotient. The
+// quotients of the force pass cannot overflow for finite operands (|grad| <=
+// |spiky| h^2, |visc term| <= |dv| visc h / 328 with density >= its self term),
+// and infinities and NaNs propagate alike; only quotients that would be
+// subnormal take the IEEE division itself.
+__device__ __forceinline__ bool subnormal_quot(double q)
+{
+    return fabs(q) < 0x1p-126 && q != 0.0;
+}
+
+// 1/b to within one rounding: hardware estimate plus two Newton-Raphson steps.
+__device__ __forceinline__ double rcp_refined(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ void div3(float a0, float a1, float a2, float b, double rb, float &q0, float &q1,
+                                     float &q2)
+{
+    const double d0 = (double) a0 * rb, d1 = (double) a1 * rb, d2 = (double) a2 * rb;
+    q0 = (float) d0;
+    q1 = (float) d1;
+    q2 = (float) d2;
+    if (__builtin_expect(subnormal_quot(d0) | subnormal_quot(d1) | subnormal_quot(d2), 0)) {
+        q0 = a0 / b;
+        q1 = a1 / b;
+        q2 = a2 / b;
+    }
+}
+
+// updateDensityPressure, src/CCPUParticleSimulator.cpp:93-141. Also stores
+// p/rho^2 (double, as src/CCPUParticleSimulator.cpp:181 forms it), 1/rho
+// (double) and, with kLists, the particle's accepted neighbours (minus itself)
+// for the force pass.
+template <bool kLists>
+__global__ __launch_bounds__(kNeighbourThreads) void k_density(ParticleSoA p, const uint32_t *__restrict__ cell,
+                                                               const uint32_t *__restrict__ start, DensityFields f,
+                                                               NeighbourList nl, uint32_t slots, Scene s, Consts k)
+{
+    __shared__ float lst[kListCap * kNeighbourThreads];
     const uint32_t li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= s.n) return;
     const uint32_t i = s.poff + li;
+    float *my = lst + threadIdx.x;
+    uint32_t *nidx = nl.idx + i;
+    const BufView bv = buf_view(p, f, slots);
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     int x, y, z;
     cell_xyz(cell[i], s, x, y, z);
-    const int xlo = x > 0 ? x - 1 : 0, xhi = x + 1 < s.rx ? x + 1 : s.rx - 1;
     float density = 0.0f;
-    uint32_t nc = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int zz = z + dz;
-        if (zz < 0 || zz >= s.rz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = y + dy;
-            if (yy < 0 || yy >= s.ry) continue;
-            const uint32_t row = s.coff + (uint32_t) (yy * s.rx + zz * s.rx * s.ry);
-            const uint32_t a = start[row + xlo], b = start[row + xhi + 1];
-            for (uint32_t j = a; j < b; ++j) {
-                const float dx = xi - p.px[j], dyv = yi - p.py[j], dzv = zi - p.pz[j];
-                const float r2 = dx * dx + dyv * dyv + dzv * dzv;
-                if (r2 <= k.hsq_f) {
-                    const double wpoly = k.poly6 * cube_cr(k.hsq - (double) r2);
-                    density = (float) ((double) density + wpoly);
-                    ++nc;
-                }
+    uint32_t nc = 0, nn = 0;
+    int cnt = 0;
+    walk_neighbours(
+        s, k, start, bv, x, y, z, xi, yi, zi, cnt,
+        [&](uint32_t j, bool valid, float xj, float yj, float zj) {
+            const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
+            const float r2 = dx * dx + dyv * dyv + dzv * dzv;
+            const bool ok = valid && r2 <= k.hsq_f;
+            my[cnt * kNeighbourThreads] = r2;
+            cnt += ok;
+            nc += ok;
+            if (kLists && ok && j != i) {
+                if (nn < nl.kmax) nidx[(size_t) nn * nl.stride] = j;
+                ++nn;
             }
-        }
-    }
+        },
+        [&]() {
+            for (int q = 0; q < cnt; ++q) {
+                const double wpoly = k.poly6 * cube_cr(k.hsq - (double) my[q * kNeighbourThreads]);
+                density = (float) ((double) density + wpoly);
+            }
+        });
     density = density * k.mass;
     const float pressure = k.gas_k * (density - k.rho0);
-    rho[i] = density;
-    // p / pow(rho, 2) in double (src/CCPUParticleSimulator.cpp:181); rho*rho is exact in double.
-    prr[i] = (double) pressure / ((double) density * (double) density);
-    if (ncount) ncount[i] = nc;
+    f.rho[i] = density;
+    // rho*rho is exact in double, so this is p / pow(rho, 2)
+    const double prr = (double) pressure / ((double) density * (double) density);
+    const double rinv = 1.0 / (double) density;
+    f.prr[i] = prr;
+    f.rinv[i] = rinv;
+    float4 *rec = f.rec + 4 * (size_t) i;
+    rec[0] = make_float4(xi, yi, zi, p.vx[i]);
+    rec[1] = make_float4(p.vy[i], p.vz[i], 0.0f, 0.0f);
+    const uint64_t pb = __builtin_bit_cast(uint64_t, prr), rb = __builtin_bit_cast(uint64_t, rinv);
+    rec[2] = make_float4(__uint_as_float((uint32_t) pb), __uint_as_float((uint32_t) (pb >> 32)),
+                         __uint_as_float((uint32_t) rb), __uint_as_float((uint32_t) (rb >> 32)));
+    if (kLists) nl.cnt[i] = nn;
+    if (f.ncount) f.ncount[i] = nc;
 }
 
-// Pressure + viscosity sums of updateForces (src/CCPUParticleSimulator.cpp:155-193)
-// followed by the acceleration (f_p + f_v + f_g) / rho.
-__device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *__restrict__ cell,
-                                          const uint32_t *__restrict__ start, const float *__restrict__ rho,
-                                          const double *__restrict__ prr, uint32_t i, const Scene &s,
-                                          const Consts &k, float xi, float yi, float zi, float vxi, float vyi,
-                                          float vzi, float &ax, float &ay, float &az)
+struct ForceSums {
+    float px, py, pz, vx, vy, vz;
+};
+
+// The neighbour data one force pair reads, loaded one pair ahead of its use so
+// that the loads of pair q+1 are in flight while pair q is computed.
+struct PairData {
+    uint32_t j;
+    float x, y, z, vx, vy, vz;
+    double prr, rinv;
+};
+
+__device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
 {
-    int x, y, z;
-    cell_xyz(cell[i], s, x, y, z);
-    const int xlo = x > 0 ? x - 1 : 0, xhi = x + 1 < s.rx ? x + 1 : s.rx - 1;
-    const float rhoi = rho[i];
-    const double ai = prr[i];
-    float fpx = 0.0f, fpy = 0.0f, fpz = 0.0f, fvx = 0.0f, fvy = 0.0f, fvz = 0.0f;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int zz = z + dz;
-        if (zz < 0 || zz >= s.rz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = y + dy;
-            if (yy < 0 || yy >= s.ry) continue;
-            const uint32_t row = s.coff + (uint32_t) (yy * s.rx + zz * s.rx * s.ry);
-            const uint32_t a = start[row + xlo], b = start[row + xhi + 1];
-            for (uint32_t j = a; j < b; ++j) {
-                const float dx = xi - p.px[j], dyv = yi - p.py[j], dzv = zi - p.pz[j];
-                const float r2 = dx * dx + dyv * dyv + dzv * dzv;
-                if (r2 <= k.hsq_f && j != i) {
-                    // WspikyGradient / WviscosityLaplacian, src/CCPUParticleSimulator.cpp:17-30
-                    const double r = sqrt((double) r2);
-                    const double hr = k.h - r;
-                    const float sg = (float) (k.spiky * (hr * hr));
-                    const float rf = (float) r;
-                    const float gx = (dx * sg) / rf, gy = (dyv * sg) / rf, gz = (dzv * sg) / rf;
-                    const float pf = (float) (ai + prr[j]);
-                    fpx = fpx + gx * pf;
-                    fpy = fpy + gy * pf;
-                    fpz = fpz + gz * pf;
-                    const float lap = (float) (k.visc * hr);
-                    const float rj = rho[j];
-                    fvx = fvx + ((p.vx[j] - vxi) * lap) / rj;
-                    fvy = fvy + ((p.vy[j] - vyi) * lap) / rj;
-                    fvz = fvz + ((p.vz[j] - vzi) * lap) / rj;
-                }
+    const uint32_t o = j * 64u;
+    const float4 a = ld4(bv.rec, o), b = ld4(bv.rec, o + 16u), c = ld4(bv.rec, o + 32u);
+    PairData d;
+    d.j = j;
+    d.x = a.x;
+    d.y = a.y;
+    d.z = a.z;
+    d.vx = a.w;
+    d.vy = b.x;
+    d.vz = b.y;
+    d.prr = __builtin_bit_cast(double, (uint64_t) __float_as_uint(c.x) | ((uint64_t) __float_as_uint(c.y) << 32));
+    d.rinv = __builtin_bit_cast(double, (uint64_t) __float_as_uint(c.z) | ((uint64_t) __float_as_uint(c.w) << 32));
+    return d;
+}
+
+// One accepted pair of updateForces, src/CCPUParticleSimulator.cpp:174-183, with
+// WspikyGradient / WviscosityLaplacian (src/CCPUParticleSimulator.cpp:17-30).
+__device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, const PairData &n, float xi, float yi,
+                                           float zi, float vxi, float vyi, float vzi, double ai, ForceSums &a)
+{
+    const float dx = xi - n.x, dyv = yi - n.y, dzv = zi - n.z;
+    const float r2 = dx * dx + dyv * dyv + dzv * dzv;
+    const double r = sqrt((double) r2);
+    const double hr = k.h - r;
+    const float sg = (float) (k.spiky * (hr * hr));
+    const float rf = (float) r;
+    float gx, gy, gz;
+    div3(dx * sg, dyv * sg, dzv * sg, rf, rcp_refined((double) rf), gx, gy, gz);
+    const float pf = (float) (ai + n.prr);
+    a.px = a.px + gx * pf;
+    a.py = a.py + gy * pf;
+    a.pz = a.pz + gz * pf;
+    const float lap = (float) (k.visc * hr);
+    float tx, ty, tz;
+    const float t0 = (n.vx - vxi) * lap, t1 = (n.vy - vyi) * lap, t2 = (n.vz - vzi) * lap;
+    const double d0 = (double) t0 * n.rinv, d1 = (double) t1 * n.rinv, d2 = (double) t2 * n.rinv;
+    tx = (float) d0;
+    ty = (float) d1;
+    tz = (float) d2;
+    if (__builtin_expect(subnormal_quot(d0) | subnormal_quot(d1) | subnormal_quot(d2), 0)) {
+        const float rj = ldf(bv.rho, n.j * 4u);
+        tx = t0 / rj;
+        ty = t1 / rj;
+        tz = t2 / rj;
+    }
+    a.vx = a.vx + tx;
+    a.vy = a.vy + ty;
+    a.vz = a.vz + tz;
+}
+
+// The force sums of particle i followed by the acceleration (f_p + f_v + f_g) / rho
+// (src/CCPUParticleSimulator.cpp:155-193). Neighbours come from the density pass's
+// list when one was kept; otherwise (or on overflow) the cells are walked again
+// in the same order.
+__device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *__restrict__ cell,
+                                          const uint32_t *__restrict__ start, const DensityFields &f,
+                                          const NeighbourList &nl, uint32_t *my, uint32_t i, uint32_t slots,
+                                          const Scene &s, const Consts &k, float xi, float yi, float zi, float vxi,
+                                          float vyi, float vzi, float &ax, float &ay, float &az)
+{
+    const BufView bv = buf_view(p, f, slots);
+    const float rhoi = f.rho[i];
+    const double ai = f.prr[i];
+    ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t cnt_i = nl.kmax ? nl.cnt[i] : 0u;
+    if (nl.kmax && cnt_i <= nl.kmax) {
+        const uint32_t *nidx = nl.idx + i;
+        uint32_t j2 = cnt_i > 1 ? nidx[nl.stride] : 0u;
+        PairData nxt = load_pair(bv, cnt_i ? nidx[0] : 0u);
+        for (uint32_t q = 0; q < cnt_i; ++q) {
+            const PairData cur = nxt;
+            if (q + 1 < cnt_i) {
+                nxt = load_pair(bv, j2);
+                if (q + 2 < cnt_i) j2 = nidx[(size_t) (q + 2) * nl.stride];
             }
+            force_pair(bv, k, cur, xi, yi, zi, vxi, vyi, vzi, ai, a);
         }
+    } else {
+        int x, y, z;
+        cell_xyz(cell[i], s, x, y, z);
+        int cnt = 0;
+        walk_neighbours(
+            s, k, start, bv, x, y, z, xi, yi, zi, cnt,
+            [&](uint32_t j, bool valid, float xj, float yj, float zj) {
+                const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
+                const float r2 = dx * dx + dyv * dyv + dzv * dzv;
+                my[cnt * kNeighbourThreads] = j;
+                cnt += valid && r2 <= k.hsq_f && j != i;
+            },
+            [&]() {
+                if (cnt == 0) return;
+                PairData nxt = load_pair(bv, my[0]);
+                for (int q = 0; q < cnt; ++q) {
+                    const PairData cur = nxt;
+                    if (q + 1 < cnt) nxt = load_pair(bv, my[(q + 1) * kNeighbourThreads]);
+                    force_pair(bv, k, cur, xi, yi, zi, vxi, vyi, vzi, ai, a);
+                }
+            });
     }
     const float fac = k.neg_mass * rhoi;
-    fpx = fpx * fac; fpy = fpy * fac; fpz = fpz * fac;
-    fvx = fvx * k.mu_mass; fvy = fvy * k.mu_mass; fvz = fvz * k.mu_mass;
+    const float fpx = a.px * fac, fpy = a.py * fac, fpz = a.pz * fac;
+    const float fvx = a.vx * k.mu_mass, fvy = a.vy * k.mu_mass, fvz = a.vz * k.mu_mass;
     const float fgx = s.gx * rhoi, fgy = s.gy * rhoi, fgz = s.gz * rhoi;
     ax = ((fpx + fvx) + fgx) / rhoi;
     ay = ((fpy + fvy) + fgy) / rhoi;
@@ -376,17 +651,18 @@ __device__ __forceinline__ void integrate_one(float xi, float yi, float zi, floa
 }
 
 __global__ __launch_bounds__(kNeighbourThreads) void k_forces(ParticleSoA p, const uint32_t *__restrict__ cell,
-                                                              const uint32_t *__restrict__ start,
-                                                              const float *__restrict__ rho,
-                                                              const double *__restrict__ prr, float *__restrict__ ax,
+                                                              const uint32_t *__restrict__ start, DensityFields f,
+                                                              NeighbourList nl, uint32_t slots, float *__restrict__ ax,
                                                               float *__restrict__ ay, float *__restrict__ az,
                                                               Scene s, Consts k)
 {
+    __shared__ uint32_t lst[kListCap * kNeighbourThreads];
     const uint32_t li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= s.n) return;
     const uint32_t i = s.poff + li;
     float a0, a1, a2;
-    sph_accel(p, cell, start, rho, prr, i, s, k, p.px[i], p.py[i], p.pz[i], p.vx[i], p.vy[i], p.vz[i], a0, a1, a2);
+    sph_accel(p, cell, start, f, nl, lst + threadIdx.x, i, slots, s, k, p.px[i], p.py[i], p.pz[i], p.vx[i],
+              p.vy[i], p.vz[i], a0, a1, a2);
     ax[i] = a0;
     ay[i] = a1;
     az[i] = a2;
@@ -424,17 +700,18 @@ __global__ void k_integrate(ParticleSoA p, const float *__restrict__ ax, const f
 // its new position for the next step's sort.
 template <bool kStoreAcc>
 __global__ __launch_bounds__(kNeighbourThreads) void k_forces_fused(
-    ParticleSoA p, const uint32_t *__restrict__ cell, const uint32_t *__restrict__ start,
-    const float *__restrict__ rho, const double *__restrict__ prr, ParticleSoA out, uint32_t *__restrict__ key_next,
-    float *__restrict__ ax, float *__restrict__ ay, float *__restrict__ az, Scene s, Consts k)
+    ParticleSoA p, const uint32_t *__restrict__ cell, const uint32_t *__restrict__ start, DensityFields f,
+    NeighbourList nl, uint32_t slots, ParticleSoA out, uint32_t *__restrict__ key_next, float *__restrict__ ax,
+    float *__restrict__ ay, float *__restrict__ az, Scene s, Consts k)
 {
+    __shared__ uint32_t lst[kListCap * kNeighbourThreads];
     const uint32_t li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= s.n) return;
     const uint32_t i = s.poff + li;
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     const float vxi = p.vx[i], vyi = p.vy[i], vzi = p.vz[i];
     float a0, a1, a2;
-    sph_accel(p, cell, start, rho, prr, i, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1, a2);
+    sph_accel(p, cell, start, f, nl, lst + threadIdx.x, i, slots, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1, a2);
     wall_accel(xi, yi, zi, vxi, vyi, vzi, s, k, a0, a1, a2);
     if (kStoreAcc) {
         ax[i] = a0;
@@ -461,12 +738,20 @@ void launch_hash(const ParticleSoA &p, uint32_t *key, const Scene &s, double h, 
 }
 
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out,
-                       uint32_t n, int shift, uint32_t *hist, hipStream_t st)
+                       uint32_t n, int shift, uint32_t *hist, uint32_t *partials, hipStream_t st)
 {
     if (n == 0) return;
     const uint32_t nb = sort_blocks(n);
+    const uint32_t m = nb * (uint32_t) kRadixBuckets;
+    const uint32_t sb = scan_blocks(m);
     hipLaunchKernelGGL(k_radix_count, dim3(nb), dim3(kSortThreads), 0, st, keys_in, n, shift, hist, nb);
-    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, hist, nb * (uint32_t) kRadixBuckets);
+    if (sb == 1) {
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, hist, m);
+    } else {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(1024), 0, st, (const uint32_t *) hist, m, partials);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, partials, sb);
+        hipLaunchKernelGGL(k_scan_apply, dim3(sb), dim3(1024), 0, st, hist, m, (const uint32_t *) partials);
+    }
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, st, keys_in, vals_in, keys_out, vals_out,
                        n, shift, (const uint32_t *) hist, nb);
 }
@@ -484,21 +769,24 @@ void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32
     hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256)), dim3(256), 0, st, src, dst, perm, n);
 }
 
-void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, float *rho, double *prr,
-                    uint32_t *ncount, const Scene &s, const Consts &k, hipStream_t st)
+void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
+                    const NeighbourList &nl, const Scene &s, const Consts &k, hipStream_t st)
 {
     if (s.n == 0) return;
-    hipLaunchKernelGGL(k_density, dim3(blocks_for(s.n, kNeighbourThreads)), dim3(kNeighbourThreads), 0, st, p,
-                       cell, start, rho, prr, ncount, s, k);
+    const dim3 grid(blocks_for(s.n, kNeighbourThreads)), block(kNeighbourThreads);
+    if (nl.kmax)
+        hipLaunchKernelGGL(k_density<true>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, s, k);
+    else
+        hipLaunchKernelGGL(k_density<false>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, s, k);
 }
 
-void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const float *rho,
-                   const double *prr, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
+void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
+                   const NeighbourList &nl, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
                    hipStream_t st)
 {
     if (s.n == 0) return;
     hipLaunchKernelGGL(k_forces, dim3(blocks_for(s.n, kNeighbourThreads)), dim3(kNeighbourThreads), 0, st, p, cell,
-                       start, rho, prr, ax, ay, az, s, k);
+                       start, f, nl, nl.stride, ax, ay, az, s, k);
 }
 
 void launch_collide(const ParticleSoA &p, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
@@ -516,18 +804,18 @@ void launch_integrate(const ParticleSoA &p, const float *ax, const float *ay, co
                        k);
 }
 
-void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const float *rho,
-                         const double *prr, const ParticleSoA &out, uint32_t *key_next, float *ax, float *ay,
+void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
+                         const NeighbourList &nl, const ParticleSoA &out, uint32_t *key_next, float *ax, float *ay,
                          float *az, const Scene &s, const Consts &k, hipStream_t st)
 {
     if (s.n == 0) return;
     const dim3 grid(blocks_for(s.n, kNeighbourThreads)), block(kNeighbourThreads);
     if (ax)
-        hipLaunchKernelGGL(k_forces_fused<true>, grid, block, 0, st, p, cell, start, rho, prr, out, key_next, ax, ay,
-                           az, s, k);
+        hipLaunchKernelGGL(k_forces_fused<true>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, out, key_next,
+                           ax, ay, az, s, k);
     else
-        hipLaunchKernelGGL(k_forces_fused<false>, grid, block, 0, st, p, cell, start, rho, prr, out, key_next, ax,
-                           ay, az, s, k);
+        hipLaunchKernelGGL(k_forces_fused<false>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, out,
+                           key_next, ax, ay, az, s, k);
 }
 
 }  // namespace sph

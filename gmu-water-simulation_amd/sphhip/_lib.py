@@ -16,6 +16,7 @@ SPH_OK, SPH_ERR_INVALID, SPH_ERR_CAPACITY, SPH_ERR_HIP, SPH_ERR_OOM, SPH_ERR_NO_
     SPH_ERR_FAULT = range(8)
 PHASE_GRID, PHASE_DENSITY, PHASE_FORCES, PHASE_COLLISIONS, PHASE_INTEGRATE = range(5)
 FLAG_DIAGNOSTICS, FLAG_PROFILE = 1, 2
+NEIGHBOUR_LISTS_OFF = 0xFFFFFFFF
 (DEBUG_CELL_PRESORT, DEBUG_PERM, DEBUG_CELL, DEBUG_CELL_START, DEBUG_NEIGHBOURS, DEBUG_DENSITY,
  DEBUG_PRESSURE, DEBUG_ACCEL, DEBUG_IDS) = range(9)
 STAGES = ["hash", "sort", "cells", "reorder", "density", "forces", "collisions", "integrate"]
@@ -49,7 +50,7 @@ SIGNATURES = {
 
 class sph_params(C.Structure):
     _fields_ = [("box", C.c_float * 3), ("gravity", C.c_float * 3), ("dt", C.c_float),
-                ("capacity", C.c_uint32), ("flags", C.c_uint32)]
+                ("capacity", C.c_uint32), ("flags", C.c_uint32), ("max_neighbours", C.c_uint32)]
 
 
 class SphError(RuntimeError):

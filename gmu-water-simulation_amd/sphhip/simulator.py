@@ -24,7 +24,7 @@ H = np.float32(0.0457)
 class HIPGridParticleSimulator:
     """Drop-in for CGPUParticleSimulator (src/CGPUParticleSimulator.cpp)."""
 
-    def __init__(self, box_size, scenario=DAM_BREAK, capacity=None, flags=0, fused=True):
+    def __init__(self, box_size, scenario=DAM_BREAK, capacity=None, flags=0, fused=True, max_neighbours=0):
         lib = _lib.load()
         self._lib = lib
         box = np.broadcast_to(np.asarray(box_size, np.float32), (3,)).astype(np.float32)
@@ -33,6 +33,7 @@ class HIPGridParticleSimulator:
         self.gravity = np.array([0.0, GRAVITY_ACCELERATION, 0.0], np.float32)
         self.fused = fused
         self.flags = int(flags)
+        self.max_neighbours = int(max_neighbours)
         self._ctx = None
         self._capacity = capacity
         self.m_particlesCount = 0
@@ -49,6 +50,7 @@ class HIPGridParticleSimulator:
             p.box[a] = float(self.box[a])
             p.gravity[a] = float(self.gravity[a])
         p.flags = self.flags
+        p.max_neighbours = self.max_neighbours
         ctx = C.c_void_p()
         _lib.check(self._lib.sph_create(C.byref(p), C.byref(ctx)))
         self._ctx = ctx

@@ -85,7 +85,12 @@ typedef struct {
     float dt;           /* 0.01f in the reference (src/CBaseParticleSimulator.cpp:7) */
     uint32_t capacity;  /* maximum particle count (m_maxParticlesCount) */
     uint32_t flags;     /* SPH_FLAG_* */
+    uint32_t max_neighbours; /* per-particle neighbour-list capacity of the force pass; 0 = default,
+                                SPH_NEIGHBOUR_LISTS_OFF = the force pass walks the cells itself.
+                                Particles with more neighbours are still exact, only slower. */
 } sph_params;
+
+#define SPH_NEIGHBOUR_LISTS_OFF 0xFFFFFFFFu
 
 /* Fill p with the reference defaults for a cubic box of edge `box` and the given
  * capacity (CBaseParticleSimulator ctor, src/CBaseParticleSimulator.cpp:3-36). */

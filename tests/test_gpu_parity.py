@@ -34,9 +34,9 @@ def assert_bits_equal(a, b, what):
                              (what, len(diff), a.size, k, a.ravel()[k], b.ravel()[k]))
 
 
-def gpu_sim(hip, box, pos, vel=None, diag=True, capacity=None):
+def gpu_sim(hip, box, pos, vel=None, diag=True, capacity=None, max_neighbours=0):
     flags = hip._lib.FLAG_DIAGNOSTICS if diag else 0
-    sim = hip.HIPGridParticleSimulator(box, flags=flags, capacity=capacity or len(pos))
+    sim = hip.HIPGridParticleSimulator(box, flags=flags, capacity=capacity or len(pos), max_neighbours=max_neighbours)
     sim.upload(pos, vel)
     return sim
 
@@ -208,6 +208,19 @@ def test_reference_fountain_appends(hip, orc):
     p, v, _ = sim.download()
     assert_bits_equal(p, o.state()["pos"], "fountain positions")
     assert_bits_equal(v, o.state()["vel"], "fountain velocities")
+
+
+def test_neighbour_list_overflow_is_exact(hip, orc):
+    """Particles whose neighbour list overflows (capacity 4 here, about 30 needed)
+    re-walk their cells in the force pass: still bitwise the oracle."""
+    pos = lattice_4k(hip)
+    sim = gpu_sim(hip, 0.4, pos, diag=False, max_neighbours=4)
+    o = oracle_sim(orc, 0.4, pos)
+    sim.advance(8)
+    o.step(8)
+    p, v, _ = sim.download()
+    assert_bits_equal(p, o.state()["pos"], "positions with overflowing lists")
+    assert_bits_equal(v, o.state()["vel"], "velocities with overflowing lists")
 
 
 def test_determinism_repeat_runs(hip):
