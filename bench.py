@@ -65,15 +65,50 @@ def cpu_baseline(pos, box, steps):
             "seconds": dt, "host": model, "host_threads": ncpu}
 
 
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of a known size, so that the FETCH_SIZE /
+    WRITE_SIZE counters of a profiled run can be scaled to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of each kernel from the committed counter summary
+    (profiles/traffic_<workload>.json, FETCH_SIZE + WRITE_SIZE scaled by a copy
+    of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="dam_break_1m")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (scales HBM counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (a scene would be "
@@ -114,8 +149,10 @@ def main():
     dom_ms = stages[dominant]
     achieved = ALG_BYTES[dominant] * n / (dom_ms * 1e-3) / 1e9
     step_bytes = sum(ALG_BYTES.values()) * n + ALG_BYTES_PER_CELL * ncells
+    traffic = load_traffic(args.workload)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic["kernels"].get(dominant, {}).get("bytes_per_launch") if traffic else None,
                 "avg_launch_ms": dom_ms, "alg_bytes_per_particle": ALG_BYTES[dominant],
                 "step_alg_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9}
     prof.close()

@@ -36,7 +36,6 @@ sph_status fail(sph_status code, const std::string &msg)
     } while (0)
 
 constexpr int kProfileSlots = 256;  // event pairs buffered before a forced drain
-constexpr uint32_t kDefaultMaxNeighbours = 192;
 
 }  // namespace
 
@@ -456,8 +455,7 @@ sph_status sph_create(const sph_params *params, sph_ctx **out)
     c->prm = *params;
     c->device = dev;
     c->cap = params->capacity;
-    c->kmax = params->max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0u
-              : params->max_neighbours ? params->max_neighbours : kDefaultMaxNeighbours;
+    c->kmax = params->max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0u : params->max_neighbours;
     c->ncells = (uint32_t) ncells;
     c->k = sph::make_consts();
     c->scene.rx = rx;

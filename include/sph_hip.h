@@ -85,9 +85,10 @@ typedef struct {
     float dt;           /* 0.01f in the reference (src/CBaseParticleSimulator.cpp:7) */
     uint32_t capacity;  /* maximum particle count (m_maxParticlesCount) */
     uint32_t flags;     /* SPH_FLAG_* */
-    uint32_t max_neighbours; /* per-particle neighbour-list capacity of the force pass; 0 = default,
-                                SPH_NEIGHBOUR_LISTS_OFF = the force pass walks the cells itself.
-                                Particles with more neighbours are still exact, only slower. */
+    uint32_t max_neighbours; /* 0 (default) or SPH_NEIGHBOUR_LISTS_OFF: the force pass walks the cells
+                                itself. N > 0: the density pass keeps lists of up to N neighbours per
+                                particle for the force pass (N*4 bytes per particle of HBM); particles
+                                with more neighbours are still exact, only slower. */
 } sph_params;
 
 #define SPH_NEIGHBOUR_LISTS_OFF 0xFFFFFFFFu

@@ -223,6 +223,19 @@ def test_neighbour_list_overflow_is_exact(hip, orc):
     assert_bits_equal(v, o.state()["vel"], "velocities with overflowing lists")
 
 
+def test_neighbour_list_mode_is_exact(hip, orc):
+    """With neighbour lists kept by the density pass (capacity 256, no overflow)
+    the force pass reads them instead of walking the cells: bitwise the oracle."""
+    pos = lattice_4k(hip)
+    sim = gpu_sim(hip, 0.4, pos, diag=False, max_neighbours=256)
+    o = oracle_sim(orc, 0.4, pos)
+    sim.advance(8)
+    o.step(8)
+    p, v, _ = sim.download()
+    assert_bits_equal(p, o.state()["pos"], "positions with neighbour lists")
+    assert_bits_equal(v, o.state()["vel"], "velocities with neighbour lists")
+
+
 def test_determinism_repeat_runs(hip):
     """Two runs from the same state are bitwise equal (no atomics in sums)."""
     pos = lattice_4k(hip)
