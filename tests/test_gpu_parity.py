@@ -308,6 +308,18 @@ def test_error_paths(hip):
     assert e.value.status == hip._lib.SPH_ERR_CAPACITY
 
 
+def test_injected_fault_is_reported(hip, monkeypatch):
+    """SPH_FAULT_AT_STEP=k makes step k fail on the host with SPH_ERR_FAULT before
+    any launch (the failure path of the caller, without a device fault)."""
+    monkeypatch.setenv("SPH_FAULT_AT_STEP", "3")
+    sim = gpu_sim(hip, 0.4, lattice_4k(hip), diag=False)
+    sim.advance(3)
+    with pytest.raises(hip.SphError) as e:
+        sim.advance(1)
+    assert e.value.status == hip._lib.SPH_ERR_FAULT
+    assert "step 3" in str(e.value)
+
+
 def test_full_size_one_step(hip, orc):
     """The 1,048,576-particle lattice (box 3.2 m, 71^3 cells): one step through
     the phases, bit-exact against the canonical oracle; sort invariants."""
