@@ -1,8 +1,13 @@
 // sph_capi.hip -- the C ABI of libsph_hip.so (declared in include/sph_hip.h).
 //
-// A context owns one scene's device state: SoA particle buffers (ping-pong), the
-// sort buffers, the cell table and one HIP stream. Every entry point reports
-// failures as a status plus a thread-local message and never throws.
+// A context owns a pool of particle slots shared by one or more independent
+// scenes, its device buffers (SoA particle state ping-pong, sort buffers, the
+// global cell table, density fields) and one HIP stream. Each scene holds a
+// segment of slots that starts on a multiple of kGroupSlots and a segment of the
+// global cell ids (its grid plus one dead cell for unused slots); free ranges of
+// the pool are scenes without cells. The device reads the segment table, so one
+// launch per stage advances every scene. Every entry point reports failures as a
+// status plus a thread-local message and never throws.
 
 #include <hip/hip_runtime.h>
 
@@ -35,18 +40,53 @@ sph_status fail(sph_status code, const std::string &msg)
             return fail(SPH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));                 \
     } while (0)
 
-constexpr int kProfileSlots = 256;  // event pairs buffered before a forced drain
+#define SPH_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        sph_status s_ = (expr);                                                                          \
+        if (s_ != SPH_OK) return s_;                                                                     \
+    } while (0)
+
+constexpr int kProfileSlots = 256;          // event pairs buffered before a forced drain
+constexpr uint32_t kMaxSceneSlots = 1u << 26;  // 64-byte records of a scene stay below 4 GiB (buffer offsets)
+constexpr uint32_t kMaxPool = 1u << 31;
+constexpr uint32_t kNoScene = 0xFFFFFFFFu;
+
+uint32_t round_up(uint32_t v, uint32_t m)
+{
+    return (uint32_t) (((uint64_t) v + m - 1) / m * m);
+}
+
+struct SceneRec {
+    bool used = false;
+    uint32_t req_cap = 0;      // capacity asked for; the segment may be longer (group rounding)
+    sph::Scene s{};            // host copy; for a scene with an emitter, n is refreshed from the device
+    bool has_emitter = false;
+    sph::Emitter em{};
+};
 
 }  // namespace
 
 struct sph_ctx {
-    sph_params prm{};
-    sph::Scene scene{};
+    uint32_t flags = 0;
     sph::Consts k{};
-    uint32_t ncells = 0;
-    uint32_t cap = 0;
     int device = 0;
     hipStream_t stream = nullptr;
+
+    // scenes and the device segment table
+    uint32_t pool = 0;
+    std::vector<SceneRec> scenes;            // by scene id
+    std::vector<sph::Scene> table_h;         // segments in slot order (scenes and free ranges)
+    std::vector<uint32_t> table_sid;         // scene id of each segment, kNoScene for a free range
+    std::vector<uint32_t> group_h;           // segment of each group of kGroupSlots slots
+    std::vector<sph::Emitter> emit_h;        // emitters with their segment index
+    sph::Scene *table_d = nullptr;
+    uint32_t *group_d = nullptr;
+    sph::Emitter *emit_d = nullptr;
+    uint32_t total_cells = 0;                // cells of all segments (grid cells + one dead cell each)
+    uint32_t cstart_cap = 0;                 // entries allocated in cstart
+    uint32_t nslots = 0;                     // slots the step covers
+    bool layout_dirty = true;                // table_h/device table must be rebuilt before the next launch
+    bool counts_stale = false;               // emitters ran since the host copies of n were refreshed
 
     // particle state: pos/vel ping-pong and ids, in the current array order
     float *pos[2][3] = {};
@@ -68,15 +108,20 @@ struct sph_ctx {
     float4 *rec = nullptr;
     float *acc[3] = {};
     uint32_t *ncount = nullptr;
-    uint32_t *nlist = nullptr;   // neighbour lists, max_neighbours x capacity
+    uint32_t *nlist = nullptr;   // neighbour lists, max_neighbours x pool
     uint32_t *nlist_cnt = nullptr;
-    uint32_t kmax = 0;
+    uint32_t kmax = 0;           // list capacity allocated
+    bool use_lists = false;
+    int variant = 0;
 
     bool keys_valid = false;   // key_h holds the cells of the current positions
-    bool grid_valid = false;   // sorted order / cell table match the current positions
     int next_phase = SPH_PHASE_GRID;
     uint64_t steps_done = 0;
     long fault_at_step = -1;
+
+    // graphs: one executable step per (cur, id_cur) parity
+    bool use_graphs = false;
+    hipGraphExec_t graph[2][2] = {};
 
     // profiling
     hipEvent_t ev[kProfileSlots][2] = {};
@@ -104,8 +149,8 @@ struct sph_ctx {
         sph::NeighbourList nl;
         nl.idx = nlist;
         nl.cnt = nlist_cnt;
-        nl.kmax = kmax;
-        nl.stride = cap;
+        nl.kmax = use_lists ? kmax : 0u;
+        nl.stride = pool;
         return nl;
     }
 
@@ -116,6 +161,15 @@ struct sph_ctx {
         s.vx = vel[which][0]; s.vy = vel[which][1]; s.vz = vel[which][2];
         s.id = ids[idw];
         return s;
+    }
+
+    sph::SceneTable table() const
+    {
+        sph::SceneTable t;
+        t.scenes = table_d;
+        t.group_scene = table_h.size() > 1 ? group_d : nullptr;
+        t.nslots = nslots;
+        return t;
     }
 };
 
@@ -147,23 +201,50 @@ Consts make_consts()
 
 namespace {
 
+void *dalloc_raw(sph_ctx *c, size_t bytes, sph_status *st)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(bytes, 1));
+    if (e != hipSuccess) {
+        *st = fail(SPH_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+        return nullptr;
+    }
+    c->allocations.push_back(q);
+    *st = SPH_OK;
+    return q;
+}
+
 template <typename T>
 sph_status dalloc(sph_ctx *c, T **p, size_t count)
 {
-    void *q = nullptr;
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess)
-        return fail(SPH_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
-    c->allocations.push_back(q);
-    *p = (T *) q;
-    return SPH_OK;
+    sph_status st;
+    *p = (T *) dalloc_raw(c, count * sizeof(T), &st);
+    return st;
+}
+
+void dfree(sph_ctx *c, void *p)
+{
+    if (!p) return;
+    auto it = std::find(c->allocations.begin(), c->allocations.end(), p);
+    if (it != c->allocations.end()) c->allocations.erase(it);
+    (void) hipFree(p);
+}
+
+void drop_graphs(sph_ctx *c)
+{
+    for (auto &row : c->graph)
+        for (auto &g : row)
+            if (g) {
+                (void) hipGraphExecDestroy(g);
+                g = nullptr;
+            }
 }
 
 void release(sph_ctx *c)
 {
     if (!c) return;
     if (c->stream) (void) hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     for (void *p : c->allocations) (void) hipFree(p);
     c->allocations.clear();
     for (auto &pair : c->ev)
@@ -182,6 +263,21 @@ bool check_ctx(const sph_ctx *c)
         return false;
     }
     return true;
+}
+
+sph_status check_scene(const sph_ctx *c, uint32_t scene, const char *who)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    if (scene >= c->scenes.size() || !c->scenes[scene].used)
+        return fail(SPH_ERR_INVALID, std::string(who) + ": no scene " + std::to_string(scene));
+    return SPH_OK;
+}
+
+sph_status between_steps(const sph_ctx *c, const char *who)
+{
+    if (c->next_phase != SPH_PHASE_GRID)
+        return fail(SPH_ERR_STATE, std::string(who) + ": only between steps (before the grid phase)");
+    return SPH_OK;
 }
 
 // --- profiling: an event pair around every stage when SPH_FLAG_PROFILE is set.
@@ -206,7 +302,7 @@ struct StageTimer {
     sph_status st = SPH_OK;
     StageTimer(sph_ctx *ctx, int stage) : c(ctx)
     {
-        if (!(c->prm.flags & SPH_FLAG_PROFILE)) return;
+        if (!(c->flags & SPH_FLAG_PROFILE)) return;
         if (c->ev_used == kProfileSlots) st = drain_profile(c);
         if (st != SPH_OK) return;
         slot = c->ev_used++;
@@ -234,21 +330,190 @@ int key_bits(uint32_t ncells)
     return b;
 }
 
+// --- scene layout
+
+// Refreshes the host copies of the live counts that emitters change on the device.
+sph_status sync_counts(sph_ctx *c)
+{
+    if (!c->counts_stale) return SPH_OK;
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    std::vector<sph::Scene> dev(c->table_h.size());
+    SPH_HIP(hipMemcpy(dev.data(), c->table_d, dev.size() * sizeof(sph::Scene), hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < dev.size(); ++t) {
+        const uint32_t sid = c->table_sid[t];
+        if (sid != kNoScene) c->scenes[sid].s.n = dev[t].n;
+    }
+    c->counts_stale = false;
+    return SPH_OK;
+}
+
+uint32_t segment_end(const sph_ctx *c, const SceneRec &r)
+{
+    return r.s.poff + r.s.cap;
+}
+
+std::vector<uint32_t> scenes_by_slot(const sph_ctx *c)
+{
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < c->scenes.size(); ++i)
+        if (c->scenes[i].used) order.push_back(i);
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t a, uint32_t b) { return c->scenes[a].s.poff < c->scenes[b].s.poff; });
+    return order;
+}
+
+// Rebuilds the segment table from the scenes (host counts must be current) and
+// uploads it with the group map and the emitters.
+sph_status rebuild_layout(sph_ctx *c)
+{
+    SPH_TRY(sync_counts(c));
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    c->table_h.clear();
+    c->table_sid.clear();
+    c->emit_h.clear();
+    uint64_t coff = 0;
+    uint32_t pos = 0;
+    auto free_range = [&](uint32_t a, uint32_t b) {
+        sph::Scene h{};
+        h.coff = (uint32_t) coff;
+        h.poff = a;
+        h.cap = b - a;
+        c->table_h.push_back(h);
+        c->table_sid.push_back(kNoScene);
+        coff += 1;
+    };
+    uint64_t active = 0;
+    for (uint32_t sid : scenes_by_slot(c)) {
+        SceneRec &r = c->scenes[sid];
+        if (r.s.poff > pos) free_range(pos, r.s.poff);
+        r.s.coff = (uint32_t) coff;
+        if (r.has_emitter) {
+            r.em.scene = (uint32_t) c->table_h.size();
+            c->emit_h.push_back(r.em);
+        }
+        c->table_h.push_back(r.s);
+        c->table_sid.push_back(sid);
+        coff += (uint64_t) r.s.ncells + 1;
+        pos = segment_end(c, r);
+        active = std::max<uint64_t>(active, (uint64_t) r.s.poff + (r.has_emitter ? r.s.cap : r.s.n));
+    }
+    if (pos < c->pool) free_range(pos, c->pool);
+    if (coff >= 0xFFFFFFFFull) return fail(SPH_ERR_CAPACITY, "the scenes' grids exceed 2^32 - 1 cells");
+    c->total_cells = (uint32_t) coff;
+    c->nslots = (uint32_t) active;
+
+    const uint32_t ngroups = (c->pool + sph::kGroupSlots - 1) / sph::kGroupSlots;
+    c->group_h.assign(ngroups, 0);
+    size_t t = 0;
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        const uint32_t slot = g * sph::kGroupSlots;
+        while (t + 1 < c->table_h.size() && c->table_h[t].poff + c->table_h[t].cap <= slot) ++t;
+        c->group_h[g] = (uint32_t) t;
+    }
+    if (c->total_cells + 1 > c->cstart_cap) {
+        dfree(c, c->cstart);
+        c->cstart = nullptr;
+        const uint32_t want = std::max<uint32_t>(c->total_cells + 1, c->cstart_cap + c->cstart_cap / 2);
+        SPH_TRY(dalloc(c, &c->cstart, want));
+        c->cstart_cap = want;
+    }
+    SPH_HIP(hipMemcpy(c->table_d, c->table_h.data(), c->table_h.size() * sizeof(sph::Scene), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(c->group_d, c->group_h.data(), c->group_h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!c->emit_h.empty())
+        SPH_HIP(hipMemcpy(c->emit_d, c->emit_h.data(), c->emit_h.size() * sizeof(sph::Emitter),
+                          hipMemcpyHostToDevice));
+    c->layout_dirty = false;
+    c->keys_valid = false;
+    drop_graphs(c);
+    return SPH_OK;
+}
+
+sph_status ensure_layout(sph_ctx *c)
+{
+    return c->layout_dirty ? rebuild_layout(c) : SPH_OK;
+}
+
+// Marks the layout for a rebuild. Callers have refreshed the counts first.
+void touch_layout(sph_ctx *c)
+{
+    c->layout_dirty = true;
+    c->keys_valid = false;
+    drop_graphs(c);
+}
+
+sph_status init_scene(const sph_scene_desc *d, sph::Scene &s, const char *who)
+{
+    for (int a = 0; a < 3; ++a)
+        if (!(d->box[a] > 0.0f) || !std::isfinite(d->box[a]))
+            return fail(SPH_ERR_INVALID, std::string(who) + ": box extents must be positive and finite");
+    if (d->capacity == 0) return fail(SPH_ERR_INVALID, std::string(who) + ": capacity must be > 0");
+    if (d->capacity > kMaxSceneSlots)
+        return fail(SPH_ERR_INVALID, std::string(who) + ": capacity above " + std::to_string(kMaxSceneSlots));
+    if (!(d->dt > 0.0f) || !std::isfinite(d->dt)) return fail(SPH_ERR_INVALID, std::string(who) + ": dt must be > 0");
+    const int rx = sph_grid_res(d->box[0]), ry = sph_grid_res(d->box[1]), rz = sph_grid_res(d->box[2]);
+    const uint64_t ncells = (uint64_t) rx * (uint64_t) ry * (uint64_t) rz;
+    if (rx <= 0 || ry <= 0 || rz <= 0 || ncells >= (1ull << 31))
+        return fail(SPH_ERR_INVALID, std::string(who) + ": grid of " + std::to_string(ncells) + " cells is out of range");
+    s = sph::Scene{};
+    s.rx = rx;
+    s.ry = ry;
+    s.rz = rz;
+    s.ncells = (uint32_t) ncells;
+    s.bx = d->box[0];
+    s.by = d->box[1];
+    s.bz = d->box[2];
+    s.gx = d->gravity[0];
+    s.gy = d->gravity[1];
+    s.gz = d->gravity[2];
+    s.dt = d->dt;
+    return SPH_OK;
+}
+
+// First free range of the pool, on a group boundary, that holds `cap` slots.
+bool find_range(const sph_ctx *c, uint32_t cap, uint32_t &poff, uint32_t &seg)
+{
+    uint32_t pos = 0;
+    auto fits = [&](uint32_t a, uint32_t b) {
+        if ((uint64_t) a + cap > b) return false;
+        poff = a;
+        seg = std::min<uint32_t>(round_up(cap, sph::kGroupSlots), b - a);
+        return true;
+    };
+    for (uint32_t sid : scenes_by_slot(c)) {
+        const SceneRec &r = c->scenes[sid];
+        if (fits(pos, r.s.poff)) return true;
+        pos = segment_end(c, r);
+    }
+    return fits(pos, c->pool);
+}
+
+// --- stages
+
+sph_status run_emit(sph_ctx *c)
+{
+    if (c->emit_h.empty()) return SPH_OK;
+    sph::launch_emit(c->emit_d, (uint32_t) c->emit_h.size(), c->table_d, c->soa(c->cur, c->id_cur), c->key_h, c->k.h,
+                     c->stream);
+    c->counts_stale = true;
+    return launch_error("emitter launch");
+}
+
 // Grid phase: hash (unless the previous integrate already did), stable LSD radix
 // sort of (cell, slot), cell start table, and gather into sorted order.
 sph_status run_grid(sph_ctx *c)
 {
-    const uint32_t n = c->scene.n;
+    const uint32_t n = c->nslots;
+    const sph::SceneTable tab = c->table();
     sph::ParticleSoA cur = c->soa(c->cur, c->id_cur);
     if (!c->keys_valid) {
         StageTimer t(c, SPH_STAGE_HASH);
         if (t.st) return t.st;
-        sph::launch_hash(cur, c->key_h, c->scene, c->k.h, c->stream);
+        sph::launch_hash(cur, c->key_h, tab, c->k.h, c->stream);
     }
     {
         StageTimer t(c, SPH_STAGE_SORT);
         if (t.st) return t.st;
-        const int bits = key_bits(c->ncells);
+        const int bits = key_bits(c->total_cells);
         const int passes = (bits + sph::kRadixBits - 1) / sph::kRadixBits;
         const uint32_t *kin = c->key_h;
         const uint32_t *vin = nullptr;
@@ -264,7 +529,7 @@ sph_status run_grid(sph_ctx *c)
     {
         StageTimer t(c, SPH_STAGE_CELLS);
         if (t.st) return t.st;
-        sph::launch_cell_start(c->key_s, n, c->ncells, c->cstart, c->stream);
+        sph::launch_cell_start(c->key_s, n, c->total_cells, c->cstart, c->stream);
     }
     {
         StageTimer t(c, SPH_STAGE_REORDER);
@@ -272,12 +537,10 @@ sph_status run_grid(sph_ctx *c)
         sph::ParticleSoA dst = c->soa(1 - c->cur, 1 - c->id_cur);
         sph::launch_reorder(cur, dst, c->val_s, n, c->stream);
     }
-    sph_status st = launch_error("grid phase launch");
-    if (st) return st;
+    SPH_TRY(launch_error("grid phase launch"));
     c->cur = 1 - c->cur;
     c->id_cur = 1 - c->id_cur;
     c->keys_valid = false;  // key_h still holds the pre-sort cells (kept for sph_debug)
-    c->grid_valid = true;
     return SPH_OK;
 }
 
@@ -285,9 +548,9 @@ sph_status run_density(sph_ctx *c)
 {
     StageTimer t(c, SPH_STAGE_DENSITY);
     if (t.st) return t.st;
-    const bool diag = (c->prm.flags & SPH_FLAG_DIAGNOSTICS) != 0;
-    sph::launch_density(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->fields(diag), c->nlists(), c->scene,
-                        c->k, c->stream);
+    const bool diag = (c->flags & SPH_FLAG_DIAGNOSTICS) != 0;
+    sph::launch_density(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->fields(diag), c->nlists(), c->table(),
+                        c->k, c->variant, c->stream);
     return launch_error("density launch");
 }
 
@@ -295,17 +558,15 @@ sph_status run_fused_tail(sph_ctx *c)
 {
     StageTimer t(c, SPH_STAGE_FORCES);
     if (t.st) return t.st;
-    const bool diag = (c->prm.flags & SPH_FLAG_DIAGNOSTICS) != 0;
+    const bool diag = (c->flags & SPH_FLAG_DIAGNOSTICS) != 0;
     sph::ParticleSoA in = c->soa(c->cur, c->id_cur);
     sph::ParticleSoA out = c->soa(1 - c->cur, c->id_cur);
     sph::launch_forces_fused(in, c->key_s, c->cstart, c->fields(diag), c->nlists(), out, c->key_h,
-                             diag ? c->acc[0] : nullptr,
-                             diag ? c->acc[1] : nullptr, diag ? c->acc[2] : nullptr, c->scene, c->k, c->stream);
-    sph_status st = launch_error("fused forces launch");
-    if (st) return st;
+                             diag ? c->acc[0] : nullptr, diag ? c->acc[1] : nullptr, diag ? c->acc[2] : nullptr,
+                             c->table(), c->k, c->variant, c->stream);
+    SPH_TRY(launch_error("fused forces launch"));
     c->cur = 1 - c->cur;
     c->keys_valid = true;
-    c->grid_valid = false;
     return SPH_OK;
 }
 
@@ -317,16 +578,72 @@ sph_status check_fault(sph_ctx *c)
     return SPH_OK;
 }
 
+sph_status check_runnable(sph_ctx *c, const char *who)
+{
+    SPH_TRY(ensure_layout(c));
+    if (c->nslots == 0) return fail(SPH_ERR_STATE, std::string(who) + ": no particles uploaded");
+    return SPH_OK;
+}
+
+// One whole step on the stream: emitters, grid, density, fused tail.
+sph_status run_step_eager(sph_ctx *c)
+{
+    SPH_TRY(run_emit(c));
+    SPH_TRY(run_grid(c));
+    SPH_TRY(run_density(c));
+    return run_fused_tail(c);
+}
+
+// One step, replayed from a graph captured for the current buffer parity. A step
+// starts from valid keys (the previous fused tail wrote them), so the graph holds
+// no hash; the first step after a layout change runs eagerly.
+sph_status run_step(sph_ctx *c)
+{
+    const bool graphs = c->use_graphs && !(c->flags & SPH_FLAG_PROFILE) && c->keys_valid;
+    if (!graphs) return run_step_eager(c);
+    hipGraphExec_t &g = c->graph[c->cur][c->id_cur];
+    const int cur0 = c->cur, id0 = c->id_cur;
+    if (!g) {
+        SPH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        sph_status st = run_step_eager(c);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        // the capture only recorded the launches: put the host state back
+        c->cur = cur0;
+        c->id_cur = id0;
+        c->keys_valid = true;
+        if (st) {
+            if (graph) (void) hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) return fail(SPH_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+        (void) hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g = nullptr;
+            return fail(SPH_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    SPH_HIP(hipGraphLaunch(g, c->stream));
+    if (!c->emit_h.empty()) c->counts_stale = true;
+    c->id_cur = 1 - c->id_cur;   // grid flips both, the fused tail flips cur back
+    c->keys_valid = true;
+    return SPH_OK;
+}
+
 sph_status run_phase(sph_ctx *c, int phase)
 {
-    if (c->scene.n == 0) return fail(SPH_ERR_STATE, "no particles uploaded");
+    if (phase < SPH_PHASE_GRID || phase > SPH_PHASE_INTEGRATE)
+        return fail(SPH_ERR_INVALID, "unknown phase " + std::to_string(phase));
     if (phase != c->next_phase)
         return fail(SPH_ERR_STATE, "phase " + std::to_string(phase) + " called out of order (expected " +
                                        std::to_string(c->next_phase) + ")");
+    if (phase == SPH_PHASE_GRID) SPH_TRY(check_runnable(c, "sph_phase"));
     sph_status st = SPH_OK;
     switch (phase) {
     case SPH_PHASE_GRID:
         st = check_fault(c);
+        if (!st) st = run_emit(c);
         if (!st) st = run_grid(c);
         break;
     case SPH_PHASE_DENSITY:
@@ -336,14 +653,15 @@ sph_status run_phase(sph_ctx *c, int phase)
         StageTimer t(c, SPH_STAGE_FORCES);
         if (t.st) return t.st;
         sph::launch_forces(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->fields(false), c->nlists(), c->acc[0],
-                           c->acc[1], c->acc[2], c->scene, c->k, c->stream);
+                           c->acc[1], c->acc[2], c->table(), c->k, c->stream);
         st = launch_error("forces launch");
         break;
     }
     case SPH_PHASE_COLLISIONS: {
         StageTimer t(c, SPH_STAGE_COLLISIONS);
         if (t.st) return t.st;
-        sph::launch_collide(c->soa(c->cur, c->id_cur), c->acc[0], c->acc[1], c->acc[2], c->scene, c->k, c->stream);
+        sph::launch_collide(c->soa(c->cur, c->id_cur), c->acc[0], c->acc[1], c->acc[2], c->table(), c->k,
+                            c->stream);
         st = launch_error("collisions launch");
         break;
     }
@@ -352,18 +670,15 @@ sph_status run_phase(sph_ctx *c, int phase)
         if (t.st) return t.st;
         sph::ParticleSoA in = c->soa(c->cur, c->id_cur);
         sph::ParticleSoA out = c->soa(1 - c->cur, c->id_cur);
-        sph::launch_integrate(in, c->acc[0], c->acc[1], c->acc[2], out, c->key_h, c->scene, c->k, c->stream);
+        sph::launch_integrate(in, c->acc[0], c->acc[1], c->acc[2], out, c->key_h, c->table(), c->k, c->stream);
         st = launch_error("integrate launch");
         if (!st) {
             c->cur = 1 - c->cur;
             c->keys_valid = true;
-            c->grid_valid = false;
             c->steps_done++;
         }
         break;
     }
-    default:
-        return fail(SPH_ERR_INVALID, "unknown phase " + std::to_string(phase));
     }
     if (st) return st;
     c->next_phase = (phase + 1) % 5;
@@ -378,28 +693,331 @@ sph_status copy_d2h(sph_ctx *c, void *dst, const void *src, size_t bytes)
     return SPH_OK;
 }
 
-sph_status write_tail(sph_ctx *c, const float *pos3, const float *vel3, const uint32_t *ids, uint32_t first,
-                      uint32_t n)
+sph_status copy_h2d(sph_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return SPH_OK;
+    SPH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    return SPH_OK;
+}
+
+// Writes n particles into slots first.. of the current buffers; ids default to
+// id_base + index.
+sph_status write_slots(sph_ctx *c, const float *pos3, const float *vel3, const uint32_t *ids, uint32_t first,
+                       uint32_t n, uint32_t id_base)
 {
     if (n == 0) return SPH_OK;
     std::vector<float> buf((size_t) n);
     std::vector<uint32_t> ib((size_t) n);
     for (int comp = 0; comp < 3; ++comp) {
         for (uint32_t i = 0; i < n; ++i) buf[i] = pos3[3 * (size_t) i + comp];
-        SPH_HIP(hipMemcpyAsync(c->pos[c->cur][comp] + first, buf.data(), n * sizeof(float), hipMemcpyHostToDevice,
-                               c->stream));
-        SPH_HIP(hipStreamSynchronize(c->stream));
+        SPH_TRY(copy_h2d(c, c->pos[c->cur][comp] + first, buf.data(), n * sizeof(float)));
         for (uint32_t i = 0; i < n; ++i) buf[i] = vel3 ? vel3[3 * (size_t) i + comp] : 0.0f;
-        SPH_HIP(hipMemcpyAsync(c->vel[c->cur][comp] + first, buf.data(), n * sizeof(float), hipMemcpyHostToDevice,
-                               c->stream));
-        SPH_HIP(hipStreamSynchronize(c->stream));
+        SPH_TRY(copy_h2d(c, c->vel[c->cur][comp] + first, buf.data(), n * sizeof(float)));
     }
-    for (uint32_t i = 0; i < n; ++i) ib[i] = ids ? ids[i] : first + i;
-    SPH_HIP(hipMemcpyAsync(c->ids[c->id_cur] + first, ib.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
-                           c->stream));
+    for (uint32_t i = 0; i < n; ++i) ib[i] = ids ? ids[i] : id_base + i;
+    return copy_h2d(c, c->ids[c->id_cur] + first, ib.data(), n * sizeof(uint32_t));
+}
+
+sph_status scene_write(sph_ctx *c, uint32_t scene, const float *pos3, const float *vel3, const uint32_t *ids,
+                       uint32_t n, bool append, const char *who)
+{
+    SPH_TRY(check_scene(c, scene, who));
+    if (n > 0 && !pos3) return fail(SPH_ERR_INVALID, std::string(who) + ": null positions");
+    SPH_TRY(sync_counts(c));
+    SceneRec &r = c->scenes[scene];
+    const uint32_t base = append ? r.s.n : 0u;
+    if ((uint64_t) base + n > r.req_cap)
+        return fail(SPH_ERR_CAPACITY, std::string(who) + ": " + std::to_string((uint64_t) base + n) +
+                                          " particles exceed capacity " + std::to_string(r.req_cap));
     SPH_HIP(hipStreamSynchronize(c->stream));
+    SPH_TRY(write_slots(c, pos3, vel3, ids, r.s.poff + base, n, base));
+    r.s.n = base + n;
+    touch_layout(c);
     return SPH_OK;
 }
+
+sph_status scene_download(sph_ctx *c, uint32_t scene, float *pos3, float *vel3, uint32_t *ids, uint32_t n,
+                          int by_id, const char *who)
+{
+    SPH_TRY(check_scene(c, scene, who));
+    SPH_TRY(sync_counts(c));
+    const sph::Scene &s = c->scenes[scene].s;
+    if (n != s.n)
+        return fail(SPH_ERR_INVALID, std::string(who) + ": n = " + std::to_string(n) + " but the scene holds " +
+                                         std::to_string(s.n));
+    if (n == 0) return SPH_OK;
+    std::vector<uint32_t> id((size_t) n);
+    SPH_TRY(copy_d2h(c, id.data(), c->ids[c->id_cur] + s.poff, n * sizeof(uint32_t)));
+    if (by_id)
+        for (uint32_t i = 0; i < n; ++i)
+            if (id[i] >= n)
+                return fail(SPH_ERR_STATE, std::string(who) + ": by_id needs ids 0..n-1, found " +
+                                               std::to_string(id[i]));
+    if (ids)
+        for (uint32_t i = 0; i < n; ++i) ids[by_id ? id[i] : i] = id[i];
+    std::vector<float> buf((size_t) n);
+    for (int comp = 0; comp < 3; ++comp) {
+        if (pos3) {
+            SPH_TRY(copy_d2h(c, buf.data(), c->pos[c->cur][comp] + s.poff, n * sizeof(float)));
+            for (uint32_t i = 0; i < n; ++i) pos3[3 * (size_t) (by_id ? id[i] : i) + comp] = buf[i];
+        }
+        if (vel3) {
+            SPH_TRY(copy_d2h(c, buf.data(), c->vel[c->cur][comp] + s.poff, n * sizeof(float)));
+            for (uint32_t i = 0; i < n; ++i) vel3[3 * (size_t) (by_id ? id[i] : i) + comp] = buf[i];
+        }
+    }
+    return SPH_OK;
+}
+
+// Copies n uint32 values from dev + first and subtracts `bias` from each.
+sph_status debug_u32(sph_ctx *c, void *out, const uint32_t *dev, uint32_t first, size_t n, uint32_t bias)
+{
+    SPH_TRY(copy_d2h(c, out, dev + first, n * 4));
+    if (bias)
+        for (size_t i = 0; i < n; ++i) ((uint32_t *) out)[i] -= bias;
+    return SPH_OK;
+}
+
+sph_status scene_debug(sph_ctx *c, uint32_t scene, int kind, void *out, size_t bytes)
+{
+    SPH_TRY(check_scene(c, scene, "sph_debug"));
+    if (!out) return fail(SPH_ERR_INVALID, "sph_debug: null output");
+    SPH_TRY(sync_counts(c));
+    if (c->layout_dirty && (kind == SPH_DEBUG_CELL_START))
+        return fail(SPH_ERR_STATE, "sph_debug: the cell table is out of date (no grid phase since the last change)");
+    const sph::Scene &s = c->scenes[scene].s;
+    const uint32_t n = s.n, p0 = s.poff;
+    const size_t need_n4 = (size_t) n * 4;
+    auto need = [&](size_t b) -> sph_status {
+        if (bytes < b)
+            return fail(SPH_ERR_INVALID, "sph_debug: buffer of " + std::to_string(bytes) + " B, need " +
+                                             std::to_string(b));
+        return SPH_OK;
+    };
+    switch (kind) {
+    case SPH_DEBUG_CELL_PRESORT:
+        SPH_TRY(need(need_n4));
+        return debug_u32(c, out, c->key_h, p0, n, s.coff);
+    case SPH_DEBUG_PERM:
+        SPH_TRY(need(need_n4));
+        return debug_u32(c, out, c->val_s, p0, n, p0);
+    case SPH_DEBUG_CELL:
+        SPH_TRY(need(need_n4));
+        return debug_u32(c, out, c->key_s, p0, n, s.coff);
+    case SPH_DEBUG_CELL_START:
+        SPH_TRY(need(((size_t) s.ncells + 1) * 4));
+        return debug_u32(c, out, c->cstart, s.coff, (size_t) s.ncells + 1, p0);
+    case SPH_DEBUG_NEIGHBOURS:
+        if (!(c->flags & SPH_FLAG_DIAGNOSTICS))
+            return fail(SPH_ERR_STATE, "sph_debug: neighbour counts need SPH_FLAG_DIAGNOSTICS");
+        SPH_TRY(need(need_n4));
+        return copy_d2h(c, out, c->ncount + p0, need_n4);
+    case SPH_DEBUG_DENSITY:
+        SPH_TRY(need(need_n4));
+        return copy_d2h(c, out, c->rho + p0, need_n4);
+    case SPH_DEBUG_PRESSURE: {
+        SPH_TRY(need(need_n4));
+        SPH_TRY(copy_d2h(c, out, c->rho + p0, need_n4));
+        float *f = (float *) out;
+        for (uint32_t i = 0; i < n; ++i) f[i] = sph::kGasStiffness * (f[i] - sph::kRestDensity);
+        return SPH_OK;
+    }
+    case SPH_DEBUG_ACCEL: {
+        SPH_TRY(need(need_n4 * 3));
+        std::vector<float> buf((size_t) n);
+        float *f = (float *) out;
+        for (int a = 0; a < 3; ++a) {
+            SPH_TRY(copy_d2h(c, buf.data(), c->acc[a] + p0, need_n4));
+            for (uint32_t i = 0; i < n; ++i) f[3 * (size_t) i + a] = buf[i];
+        }
+        return SPH_OK;
+    }
+    case SPH_DEBUG_IDS:
+        SPH_TRY(need(need_n4));
+        return copy_d2h(c, out, c->ids[c->id_cur] + p0, need_n4);
+    default:
+        return fail(SPH_ERR_INVALID, "sph_debug: unknown kind " + std::to_string(kind));
+    }
+}
+
+sph_status check_device(int *dev_out)
+{
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(SPH_ERR_NO_DEVICE, "no HIP device visible");
+    int dev = 0;
+    SPH_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SPH_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SPH_ERR_NO_DEVICE, std::string("libsph_hip.so is built for gfx950, device is ") + prop.gcnArchName);
+    *dev_out = dev;
+    return SPH_OK;
+}
+
+sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
+{
+    const size_t P = c->pool;
+    for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 3; ++a) {
+            SPH_TRY(dalloc(c, &c->pos[b][a], P));
+            SPH_TRY(dalloc(c, &c->vel[b][a], P));
+        }
+    for (int b = 0; b < 2; ++b) SPH_TRY(dalloc(c, &c->ids[b], P));
+    SPH_TRY(dalloc(c, &c->key_h, P));
+    SPH_TRY(dalloc(c, &c->key_t, P));
+    SPH_TRY(dalloc(c, &c->key_s, P));
+    SPH_TRY(dalloc(c, &c->val_t, P));
+    SPH_TRY(dalloc(c, &c->val_s, P));
+    const size_t hist_n = (size_t) sph::kRadixBuckets * sph::sort_blocks(c->pool);
+    SPH_TRY(dalloc(c, &c->hist, hist_n));
+    SPH_TRY(dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n)));
+    SPH_TRY(dalloc(c, &c->rho, P));
+    SPH_TRY(dalloc(c, &c->prr, P));
+    SPH_TRY(dalloc(c, &c->rinv, P));
+    SPH_TRY(dalloc(c, &c->rec, 4 * P));
+    for (int a = 0; a < 3; ++a) SPH_TRY(dalloc(c, &c->acc[a], P));
+    SPH_TRY(dalloc(c, &c->ncount, P));
+    SPH_TRY(dalloc(c, &c->nlist_cnt, P));
+    SPH_TRY(dalloc(c, &c->nlist, (size_t) c->kmax * P));
+    SPH_TRY(dalloc(c, &c->table_d, max_segments));
+    SPH_TRY(dalloc(c, &c->group_d, (P + sph::kGroupSlots - 1) / sph::kGroupSlots));
+    SPH_TRY(dalloc(c, &c->emit_d, c->scenes.size()));
+    if (c->flags & SPH_FLAG_PROFILE)
+        for (auto &pair : c->ev)
+            for (auto &ev : pair)
+                if (hipEventCreate(&ev) != hipSuccess) return fail(SPH_ERR_HIP, "hipEventCreate failed");
+    for (auto &ev : c->span)
+        if (hipEventCreate(&ev) != hipSuccess) return fail(SPH_ERR_HIP, "hipEventCreate failed");
+    return SPH_OK;
+}
+
+// Creates a context over `pool` slots with up to max_scenes scene ids and places
+// the given scenes one after another.
+sph_status create_ctx(uint32_t pool, uint32_t max_scenes, uint32_t flags, uint32_t max_neighbours,
+                      const sph_scene_desc *descs, uint32_t nscenes, sph_ctx **out, const char *who)
+{
+    if (!out) return fail(SPH_ERR_INVALID, std::string(who) + ": null output");
+    *out = nullptr;
+    if (nscenes > 0 && !descs) return fail(SPH_ERR_INVALID, std::string(who) + ": null scenes");
+    if (max_neighbours > 4096 && max_neighbours != SPH_NEIGHBOUR_LISTS_OFF)
+        return fail(SPH_ERR_INVALID, std::string(who) + ": max_neighbours above 4096");
+    if (max_scenes == 0 || max_scenes < nscenes || max_scenes > (1u << 20))
+        return fail(SPH_ERR_INVALID, std::string(who) + ": max_scenes must be in [max(nscenes, 1), 2^20]");
+    // scenes one after another, each from a group boundary
+    std::vector<sph::Scene> placed(nscenes);
+    uint64_t end = 0;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        SPH_TRY(init_scene(&descs[i], placed[i], who));
+        if (end > kMaxPool)
+            return fail(SPH_ERR_CAPACITY, std::string(who) + ": the scenes need more than " +
+                                              std::to_string(kMaxPool) + " slots");
+        placed[i].poff = (uint32_t) end;
+        end += round_up(descs[i].capacity, sph::kGroupSlots);
+    }
+    if (pool == 0) pool = (uint32_t) std::min<uint64_t>(end, (uint64_t) kMaxPool + 1);
+    if (pool == 0 || pool > kMaxPool)
+        return fail(SPH_ERR_INVALID, std::string(who) + ": pool of " + std::to_string(pool) + " slots out of range");
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        if ((uint64_t) placed[i].poff + descs[i].capacity > pool)
+            return fail(SPH_ERR_CAPACITY, std::string(who) + ": the scenes do not fit the pool");
+        placed[i].cap = std::min<uint32_t>(round_up(descs[i].capacity, sph::kGroupSlots), pool - placed[i].poff);
+    }
+    int dev = 0;
+    SPH_TRY(check_device(&dev));
+
+    sph_ctx *c = new sph_ctx();
+    c->flags = flags;
+    c->device = dev;
+    c->pool = pool;
+    c->kmax = max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0u : max_neighbours;
+    c->use_lists = c->kmax > 0;
+    c->k = sph::make_consts();
+    c->scenes.resize(max_scenes);
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        c->scenes[i].used = true;
+        c->scenes[i].req_cap = descs[i].capacity;
+        c->scenes[i].s = placed[i];
+    }
+    if (const char *f = std::getenv("SPH_FAULT_AT_STEP")) c->fault_at_step = std::atol(f);
+
+    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        release(c);
+        return fail(SPH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    sph_status st = alloc_buffers(c, 2 * max_scenes + 1);
+    if (!st) st = rebuild_layout(c);
+    if (st) {
+        std::string msg = g_last_error;
+        release(c);
+        g_last_error = msg;
+        return st;
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+// Saved state of the autotuner: the current particle buffers, keys and counts.
+struct Snapshot {
+    sph_ctx *c;
+    void *mem = nullptr;
+    std::vector<sph::Scene> table;
+    int cur = 0, id_cur = 0;
+    bool keys_valid = false;
+    uint64_t steps_done = 0;
+
+    sph_status save()
+    {
+        const size_t P = c->pool;
+        sph_status st;
+        mem = dalloc_raw(c, P * 32, &st);
+        if (st) return st;
+        char *m = (char *) mem;
+        for (int a = 0; a < 3; ++a) {
+            SPH_HIP(hipMemcpyAsync(m + a * P * 4, c->pos[c->cur][a], P * 4, hipMemcpyDeviceToDevice, c->stream));
+            SPH_HIP(hipMemcpyAsync(m + (3 + a) * P * 4, c->vel[c->cur][a], P * 4, hipMemcpyDeviceToDevice, c->stream));
+        }
+        SPH_HIP(hipMemcpyAsync(m + 6 * P * 4, c->ids[c->id_cur], P * 4, hipMemcpyDeviceToDevice, c->stream));
+        SPH_HIP(hipMemcpyAsync(m + 7 * P * 4, c->key_h, P * 4, hipMemcpyDeviceToDevice, c->stream));
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        table = c->table_h;
+        SPH_HIP(hipMemcpy(table.data(), c->table_d, table.size() * sizeof(sph::Scene), hipMemcpyDeviceToHost));
+        cur = c->cur;
+        id_cur = c->id_cur;
+        keys_valid = c->keys_valid;
+        steps_done = c->steps_done;
+        return SPH_OK;
+    }
+
+    sph_status restore()
+    {
+        const size_t P = c->pool;
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        c->cur = cur;
+        c->id_cur = id_cur;
+        const char *m = (const char *) mem;
+        for (int a = 0; a < 3; ++a) {
+            SPH_HIP(hipMemcpyAsync(c->pos[cur][a], m + a * P * 4, P * 4, hipMemcpyDeviceToDevice, c->stream));
+            SPH_HIP(hipMemcpyAsync(c->vel[cur][a], m + (3 + a) * P * 4, P * 4, hipMemcpyDeviceToDevice, c->stream));
+        }
+        SPH_HIP(hipMemcpyAsync(c->ids[id_cur], m + 6 * P * 4, P * 4, hipMemcpyDeviceToDevice, c->stream));
+        SPH_HIP(hipMemcpyAsync(c->key_h, m + 7 * P * 4, P * 4, hipMemcpyDeviceToDevice, c->stream));
+        SPH_HIP(hipMemcpyAsync(c->table_d, table.data(), table.size() * sizeof(sph::Scene), hipMemcpyHostToDevice,
+                               c->stream));
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        c->keys_valid = keys_valid;
+        c->steps_done = steps_done;
+        c->counts_stale = !c->emit_h.empty();
+        return SPH_OK;
+    }
+
+    ~Snapshot()
+    {
+        if (mem) dfree(c, mem);
+    }
+};
 
 }  // namespace
 
@@ -419,6 +1037,16 @@ void sph_default_params(sph_params *p, float box, uint32_t capacity)
     p->max_neighbours = 0;
 }
 
+void sph_default_scene(sph_scene_desc *d, float box, uint32_t capacity)
+{
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->box[0] = d->box[1] = d->box[2] = box;
+    d->gravity[1] = sph::kGravityY;
+    d->dt = sph::kDt;
+    d->capacity = capacity;
+}
+
 int sph_grid_res(float box)
 {
     volatile float q = box / sph::kH;
@@ -428,96 +1056,21 @@ int sph_grid_res(float box)
 sph_status sph_create(const sph_params *params, sph_ctx **out)
 {
     if (!params || !out) return fail(SPH_ERR_INVALID, "sph_create: null argument");
-    *out = nullptr;
-    for (int a = 0; a < 3; ++a)
-        if (!(params->box[a] > 0.0f) || !std::isfinite(params->box[a]))
-            return fail(SPH_ERR_INVALID, "sph_create: box extents must be positive and finite");
+    sph_scene_desc d;
+    std::memcpy(d.box, params->box, sizeof(d.box));
+    std::memcpy(d.gravity, params->gravity, sizeof(d.gravity));
+    d.dt = params->dt;
+    d.capacity = params->capacity;
     if (params->capacity == 0) return fail(SPH_ERR_INVALID, "sph_create: capacity must be > 0");
-    if (!(params->dt > 0.0f)) return fail(SPH_ERR_INVALID, "sph_create: dt must be > 0");
-    if (params->max_neighbours > 4096 && params->max_neighbours != SPH_NEIGHBOUR_LISTS_OFF)
-        return fail(SPH_ERR_INVALID, "sph_create: max_neighbours above 4096");
-    const int rx = sph_grid_res(params->box[0]), ry = sph_grid_res(params->box[1]), rz = sph_grid_res(params->box[2]);
-    const uint64_t ncells = (uint64_t) rx * (uint64_t) ry * (uint64_t) rz;
-    if (ncells == 0 || ncells >= (1ull << 31))
-        return fail(SPH_ERR_INVALID, "sph_create: grid of " + std::to_string(ncells) + " cells is out of range");
+    return create_ctx(params->capacity, 1, params->flags, params->max_neighbours, &d, 1, out, "sph_create");
+}
 
-    int ndev = 0;
-    hipError_t e = hipGetDeviceCount(&ndev);
-    if (e != hipSuccess || ndev == 0) return fail(SPH_ERR_NO_DEVICE, "no HIP device visible");
-    int dev = 0;
-    SPH_HIP(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    SPH_HIP(hipGetDeviceProperties(&prop, dev));
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return fail(SPH_ERR_NO_DEVICE, std::string("libsph_hip.so is built for gfx950, device is ") + prop.gcnArchName);
-
-    sph_ctx *c = new sph_ctx();
-    c->prm = *params;
-    c->device = dev;
-    c->cap = params->capacity;
-    c->kmax = params->max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0u : params->max_neighbours;
-    c->ncells = (uint32_t) ncells;
-    c->k = sph::make_consts();
-    c->scene.rx = rx;
-    c->scene.ry = ry;
-    c->scene.rz = rz;
-    c->scene.coff = 0;
-    c->scene.poff = 0;
-    c->scene.n = 0;
-    c->scene.bx = params->box[0];
-    c->scene.by = params->box[1];
-    c->scene.bz = params->box[2];
-    c->scene.gx = params->gravity[0];
-    c->scene.gy = params->gravity[1];
-    c->scene.gz = params->gravity[2];
-    c->scene.dt = params->dt;
-    if (const char *f = std::getenv("SPH_FAULT_AT_STEP")) c->fault_at_step = std::atol(f);
-
-    sph_status st = SPH_OK;
-    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (he != hipSuccess) {
-        release(c);
-        return fail(SPH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
-    }
-    const size_t P = c->cap;
-    for (int b = 0; b < 2 && !st; ++b)
-        for (int a = 0; a < 3 && !st; ++a) {
-            st = dalloc(c, &c->pos[b][a], P);
-            if (!st) st = dalloc(c, &c->vel[b][a], P);
-        }
-    for (int b = 0; b < 2 && !st; ++b) st = dalloc(c, &c->ids[b], P);
-    if (!st) st = dalloc(c, &c->key_h, P);
-    if (!st) st = dalloc(c, &c->key_t, P);
-    if (!st) st = dalloc(c, &c->key_s, P);
-    if (!st) st = dalloc(c, &c->val_t, P);
-    if (!st) st = dalloc(c, &c->val_s, P);
-    const size_t hist_n = (size_t) sph::kRadixBuckets * sph::sort_blocks(c->cap);
-    if (!st) st = dalloc(c, &c->hist, hist_n);
-    if (!st) st = dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n));
-    if (!st) st = dalloc(c, &c->cstart, (size_t) c->ncells + 1);
-    if (!st) st = dalloc(c, &c->rho, P);
-    if (!st) st = dalloc(c, &c->prr, P);
-    if (!st) st = dalloc(c, &c->rinv, P);
-    if (!st) st = dalloc(c, &c->rec, 4 * P);
-    for (int a = 0; a < 3 && !st; ++a) st = dalloc(c, &c->acc[a], P);
-    if (!st) st = dalloc(c, &c->ncount, P);
-    if (!st) st = dalloc(c, &c->nlist_cnt, P);
-    if (!st) st = dalloc(c, &c->nlist, (size_t) c->kmax * P);
-    if (!st && (params->flags & SPH_FLAG_PROFILE)) {
-        for (auto &pair : c->ev)
-            for (auto &ev : pair)
-                if (!st && hipEventCreate(&ev) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventCreate failed");
-    }
-    for (auto &ev : c->span)
-        if (!st && hipEventCreate(&ev) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventCreate failed");
-    if (st) {
-        std::string msg = g_last_error;
-        release(c);
-        g_last_error = msg;
-        return st;
-    }
-    *out = c;
-    return SPH_OK;
+sph_status sph_batch_create(const sph_batch_params *bp, const sph_scene_desc *scenes, uint32_t nscenes,
+                            sph_ctx **out)
+{
+    if (!bp) return fail(SPH_ERR_INVALID, "sph_batch_create: null parameters");
+    const uint32_t max_scenes = bp->max_scenes ? bp->max_scenes : nscenes;
+    return create_ctx(bp->pool, max_scenes, bp->flags, bp->max_neighbours, scenes, nscenes, out, "sph_batch_create");
 }
 
 void sph_destroy(sph_ctx *ctx)
@@ -525,19 +1078,47 @@ void sph_destroy(sph_ctx *ctx)
     release(ctx);
 }
 
+sph_status sph_scene_add(sph_ctx *c, const sph_scene_desc *desc, uint32_t *scene)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    if (!desc || !scene) return fail(SPH_ERR_INVALID, "sph_scene_add: null argument");
+    SPH_TRY(between_steps(c, "sph_scene_add"));
+    sph::Scene s;
+    SPH_TRY(init_scene(desc, s, "sph_scene_add"));
+    uint32_t id = 0;
+    while (id < c->scenes.size() && c->scenes[id].used) ++id;
+    if (id == c->scenes.size()) return fail(SPH_ERR_CAPACITY, "sph_scene_add: all scene ids are in use");
+    SPH_TRY(sync_counts(c));
+    uint32_t poff = 0, seg = 0;
+    if (!find_range(c, desc->capacity, poff, seg))
+        return fail(SPH_ERR_CAPACITY, "sph_scene_add: no free range of " + std::to_string(desc->capacity) +
+                                          " slots in the pool");
+    s.poff = poff;
+    s.cap = seg;
+    SceneRec &r = c->scenes[id];
+    r = SceneRec{};
+    r.used = true;
+    r.req_cap = desc->capacity;
+    r.s = s;
+    touch_layout(c);
+    *scene = id;
+    return SPH_OK;
+}
+
+sph_status sph_scene_remove(sph_ctx *c, uint32_t scene)
+{
+    SPH_TRY(check_scene(c, scene, "sph_scene_remove"));
+    SPH_TRY(between_steps(c, "sph_scene_remove"));
+    SPH_TRY(sync_counts(c));
+    c->scenes[scene] = SceneRec{};
+    touch_layout(c);
+    return SPH_OK;
+}
+
 sph_status sph_upload(sph_ctx *c, const float *pos3, const float *vel3, const uint32_t *ids, uint32_t n)
 {
     if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (n > 0 && !pos3) return fail(SPH_ERR_INVALID, "sph_upload: null positions");
-    if (n > c->cap)
-        return fail(SPH_ERR_CAPACITY, "sph_upload: " + std::to_string(n) + " particles exceed capacity " +
-                                          std::to_string(c->cap));
-    SPH_HIP(hipStreamSynchronize(c->stream));
-    sph_status st = write_tail(c, pos3, vel3, ids, 0, n);
-    if (st) return st;
-    c->scene.n = n;
-    c->keys_valid = false;
-    c->grid_valid = false;
+    SPH_TRY(scene_write(c, 0, pos3, vel3, ids, n, false, "sph_upload"));
     c->next_phase = SPH_PHASE_GRID;
     return SPH_OK;
 }
@@ -546,27 +1127,92 @@ sph_status sph_append(sph_ctx *c, const float *pos3, const float *vel3, const ui
 {
     if (!check_ctx(c)) return SPH_ERR_INVALID;
     if (n == 0) return SPH_OK;
-    if (!pos3) return fail(SPH_ERR_INVALID, "sph_append: null positions");
-    if (c->next_phase != SPH_PHASE_GRID)
-        return fail(SPH_ERR_STATE, "sph_append: only between steps (before the grid phase)");
-    if ((uint64_t) c->scene.n + n > c->cap)
-        return fail(SPH_ERR_CAPACITY, "sph_append: " + std::to_string(c->scene.n + (uint64_t) n) +
-                                          " particles exceed capacity " + std::to_string(c->cap));
-    SPH_HIP(hipStreamSynchronize(c->stream));
-    sph_status st = write_tail(c, pos3, vel3, ids, c->scene.n, n);
-    if (st) return st;
-    c->scene.n += n;
-    c->keys_valid = false;
-    c->grid_valid = false;
+    SPH_TRY(between_steps(c, "sph_append"));
+    return scene_write(c, 0, pos3, vel3, ids, n, true, "sph_append");
+}
+
+sph_status sph_scene_upload(sph_ctx *c, uint32_t scene, const float *pos3, const float *vel3, const uint32_t *ids,
+                            uint32_t n)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    SPH_TRY(between_steps(c, "sph_scene_upload"));
+    return scene_write(c, scene, pos3, vel3, ids, n, false, "sph_scene_upload");
+}
+
+sph_status sph_scene_append(sph_ctx *c, uint32_t scene, const float *pos3, const float *vel3, const uint32_t *ids,
+                            uint32_t n)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    if (n == 0) return SPH_OK;
+    SPH_TRY(between_steps(c, "sph_scene_append"));
+    return scene_write(c, scene, pos3, vel3, ids, n, true, "sph_scene_append");
+}
+
+sph_status sph_scene_set_gravity(sph_ctx *c, uint32_t scene, float gx, float gy, float gz)
+{
+    SPH_TRY(check_scene(c, scene, "sph_set_gravity"));
+    SPH_TRY(sync_counts(c));
+    sph::Scene &s = c->scenes[scene].s;
+    s.gx = gx;
+    s.gy = gy;
+    s.gz = gz;
+    // only the gravity of one table entry changes: patch it in place so that the
+    // layout, the keys and captured graphs stay valid
+    if (!c->layout_dirty) {
+        for (size_t t = 0; t < c->table_sid.size(); ++t)
+            if (c->table_sid[t] == scene) {
+                c->table_h[t].gx = gx;
+                c->table_h[t].gy = gy;
+                c->table_h[t].gz = gz;
+                SPH_HIP(hipStreamSynchronize(c->stream));
+                SPH_HIP(hipMemcpy(&c->table_d[t].gx, &c->table_h[t].gx, 3 * sizeof(float), hipMemcpyHostToDevice));
+            }
+    }
     return SPH_OK;
 }
 
 sph_status sph_set_gravity(sph_ctx *c, float gx, float gy, float gz)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    c->scene.gx = gx;
-    c->scene.gy = gy;
-    c->scene.gz = gz;
+    return sph_scene_set_gravity(c, 0, gx, gy, gz);
+}
+
+sph_status sph_scene_set_emitter(sph_ctx *c, uint32_t scene, const sph_emitter *e)
+{
+    SPH_TRY(check_scene(c, scene, "sph_scene_set_emitter"));
+    SPH_TRY(between_steps(c, "sph_scene_set_emitter"));
+    SPH_TRY(sync_counts(c));
+    SceneRec &r = c->scenes[scene];
+    if (!e) {
+        r.has_emitter = false;
+        touch_layout(c);
+        return SPH_OK;
+    }
+    if (e->kind != SPH_EMIT_FOUNTAIN && e->kind != SPH_EMIT_DISC)
+        return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: unknown kind " + std::to_string(e->kind));
+    if (e->kind == SPH_EMIT_DISC) {
+        if (e->rate == 0) return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: rate must be > 0");
+        if (!(e->radius >= 0.0f) || !std::isfinite(e->radius) || !std::isfinite(e->speed) ||
+            !(e->spread >= 0.0f) || !std::isfinite(e->spread) || !std::isfinite(e->centre[0]) ||
+            !std::isfinite(e->centre[1]))
+            return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: disc parameters must be finite, radius >= 0");
+    }
+    sph::Emitter &m = r.em;
+    m = sph::Emitter{};
+    m.kind = e->kind;
+    m.rate = e->rate;
+    // never beyond the capacity the caller reserved
+    m.limit = e->kind == SPH_EMIT_DISC ? std::min(e->limit, r.req_cap) : e->limit;
+    if (e->kind == SPH_EMIT_FOUNTAIN && (e->limit > r.req_cap))
+        return fail(SPH_ERR_CAPACITY, "sph_scene_set_emitter: fountain limit above the scene's capacity");
+    m.cx = e->centre[0];
+    m.cz = e->centre[1];
+    m.radius = e->radius;
+    m.speed = e->speed;
+    m.spread = e->spread;
+    m.seed_lo = (uint32_t) e->seed;
+    m.seed_hi = (uint32_t) (e->seed >> 32);
+    r.has_emitter = true;
+    touch_layout(c);
     return SPH_OK;
 }
 
@@ -574,8 +1220,7 @@ sph_status sph_phase(sph_ctx *c, int phase, float *ms_out)
 {
     if (!check_ctx(c)) return SPH_ERR_INVALID;
     if (ms_out) SPH_HIP(hipEventRecord(c->span[0], c->stream));
-    sph_status st = run_phase(c, phase);
-    if (st) return st;
+    SPH_TRY(run_phase(c, phase));
     if (ms_out) {
         SPH_HIP(hipEventRecord(c->span[1], c->stream));
         SPH_HIP(hipEventSynchronize(c->span[1]));
@@ -587,15 +1232,12 @@ sph_status sph_phase(sph_ctx *c, int phase, float *ms_out)
 sph_status sph_step(sph_ctx *c, uint32_t nsteps, float *ms_out)
 {
     if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (c->scene.n == 0) return fail(SPH_ERR_STATE, "sph_step: no particles uploaded");
     if (c->next_phase != SPH_PHASE_GRID) return fail(SPH_ERR_STATE, "sph_step: a phase sequence is in progress");
+    SPH_TRY(check_runnable(c, "sph_step"));
     if (ms_out) SPH_HIP(hipEventRecord(c->span[0], c->stream));
     for (uint32_t s = 0; s < nsteps; ++s) {
-        sph_status st = check_fault(c);
-        if (!st) st = run_grid(c);
-        if (!st) st = run_density(c);
-        if (!st) st = run_fused_tail(c);
-        if (st) return st;
+        SPH_TRY(check_fault(c));
+        SPH_TRY(run_step(c));
         c->steps_done++;
     }
     if (ms_out) {
@@ -608,116 +1250,170 @@ sph_status sph_step(sph_ctx *c, uint32_t nsteps, float *ms_out)
 
 sph_status sph_download(sph_ctx *c, float *pos3, float *vel3, uint32_t *ids, uint32_t n, int by_id)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (n != c->scene.n)
-        return fail(SPH_ERR_INVALID, "sph_download: n = " + std::to_string(n) + " but the context holds " +
-                                         std::to_string(c->scene.n));
-    if (n == 0) return SPH_OK;
-    std::vector<uint32_t> id((size_t) n);
-    sph_status st = copy_d2h(c, id.data(), c->ids[c->id_cur], n * sizeof(uint32_t));
-    if (st) return st;
-    if (by_id)
-        for (uint32_t i = 0; i < n; ++i)
-            if (id[i] >= n)
-                return fail(SPH_ERR_STATE, "sph_download: by_id needs ids 0..n-1, found " + std::to_string(id[i]));
-    if (ids) {
-        for (uint32_t i = 0; i < n; ++i) ids[by_id ? id[i] : i] = id[i];
-    }
-    std::vector<float> buf((size_t) n);
-    for (int comp = 0; comp < 3; ++comp) {
-        if (pos3) {
-            st = copy_d2h(c, buf.data(), c->pos[c->cur][comp], n * sizeof(float));
-            if (st) return st;
-            for (uint32_t i = 0; i < n; ++i) pos3[3 * (size_t) (by_id ? id[i] : i) + comp] = buf[i];
-        }
-        if (vel3) {
-            st = copy_d2h(c, buf.data(), c->vel[c->cur][comp], n * sizeof(float));
-            if (st) return st;
-            for (uint32_t i = 0; i < n; ++i) vel3[3 * (size_t) (by_id ? id[i] : i) + comp] = buf[i];
-        }
-    }
-    return SPH_OK;
+    return scene_download(c, 0, pos3, vel3, ids, n, by_id, "sph_download");
+}
+
+sph_status sph_scene_download(sph_ctx *c, uint32_t scene, float *pos3, float *vel3, uint32_t *ids, uint32_t n,
+                              int by_id)
+{
+    return scene_download(c, scene, pos3, vel3, ids, n, by_id, "sph_scene_download");
 }
 
 sph_status sph_debug(sph_ctx *c, int kind, void *out, size_t bytes)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (!out) return fail(SPH_ERR_INVALID, "sph_debug: null output");
-    const uint32_t n = c->scene.n;
-    const size_t need_n4 = (size_t) n * 4;
-    auto need = [&](size_t b) -> sph_status {
-        if (bytes < b)
-            return fail(SPH_ERR_INVALID, "sph_debug: buffer of " + std::to_string(bytes) + " B, need " +
-                                             std::to_string(b));
-        return SPH_OK;
-    };
-    sph_status st = SPH_OK;
-    switch (kind) {
-    case SPH_DEBUG_CELL_PRESORT:
-        if ((st = need(need_n4))) return st;
-        return copy_d2h(c, out, c->key_h, need_n4);
-    case SPH_DEBUG_PERM:
-        if ((st = need(need_n4))) return st;
-        return copy_d2h(c, out, c->val_s, need_n4);
-    case SPH_DEBUG_CELL:
-        if ((st = need(need_n4))) return st;
-        return copy_d2h(c, out, c->key_s, need_n4);
-    case SPH_DEBUG_CELL_START:
-        if ((st = need(((size_t) c->ncells + 1) * 4))) return st;
-        return copy_d2h(c, out, c->cstart, ((size_t) c->ncells + 1) * 4);
-    case SPH_DEBUG_NEIGHBOURS:
-        if (!(c->prm.flags & SPH_FLAG_DIAGNOSTICS))
-            return fail(SPH_ERR_STATE, "sph_debug: neighbour counts need SPH_FLAG_DIAGNOSTICS");
-        if ((st = need(need_n4))) return st;
-        return copy_d2h(c, out, c->ncount, need_n4);
-    case SPH_DEBUG_DENSITY:
-        if ((st = need(need_n4))) return st;
-        return copy_d2h(c, out, c->rho, need_n4);
-    case SPH_DEBUG_PRESSURE: {
-        if ((st = need(need_n4))) return st;
-        st = copy_d2h(c, out, c->rho, need_n4);
-        if (st) return st;
-        float *f = (float *) out;
-        for (uint32_t i = 0; i < n; ++i) f[i] = sph::kGasStiffness * (f[i] - sph::kRestDensity);
-        return SPH_OK;
-    }
-    case SPH_DEBUG_ACCEL: {
-        if ((st = need(need_n4 * 3))) return st;
-        std::vector<float> buf((size_t) n);
-        float *f = (float *) out;
-        for (int a = 0; a < 3; ++a) {
-            st = copy_d2h(c, buf.data(), c->acc[a], need_n4);
-            if (st) return st;
-            for (uint32_t i = 0; i < n; ++i) f[3 * (size_t) i + a] = buf[i];
-        }
-        return SPH_OK;
-    }
-    case SPH_DEBUG_IDS:
-        if ((st = need(need_n4))) return st;
-        return copy_d2h(c, out, c->ids[c->id_cur], need_n4);
-    default:
-        return fail(SPH_ERR_INVALID, "sph_debug: unknown kind " + std::to_string(kind));
-    }
+    return scene_debug(c, 0, kind, out, bytes);
 }
 
-sph_status sph_count(const sph_ctx *c, uint32_t *n)
+sph_status sph_scene_debug(sph_ctx *c, uint32_t scene, int kind, void *out, size_t bytes)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    return scene_debug(c, scene, kind, out, bytes);
+}
+
+sph_status sph_count(const sph_ctx *cc, uint32_t *n)
+{
+    return sph_scene_count(const_cast<sph_ctx *>(cc), 0, n);
+}
+
+sph_status sph_scene_count(sph_ctx *c, uint32_t scene, uint32_t *n)
+{
+    SPH_TRY(check_scene(c, scene, "sph_count"));
     if (!n) return fail(SPH_ERR_INVALID, "sph_count: null output");
-    *n = c->scene.n;
+    SPH_TRY(sync_counts(c));
+    *n = c->scenes[scene].s.n;
     return SPH_OK;
 }
 
 sph_status sph_grid_dims(const sph_ctx *c, int *xyz, uint32_t *ncells)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    return sph_scene_grid(const_cast<sph_ctx *>(c), 0, xyz, ncells, nullptr, nullptr);
+}
+
+sph_status sph_scene_grid(sph_ctx *c, uint32_t scene, int *xyz, uint32_t *ncells, uint32_t *first_slot,
+                          uint32_t *slots)
+{
+    SPH_TRY(check_scene(c, scene, "sph_grid_dims"));
+    const sph::Scene &s = c->scenes[scene].s;
     if (xyz) {
-        xyz[0] = c->scene.rx;
-        xyz[1] = c->scene.ry;
-        xyz[2] = c->scene.rz;
+        xyz[0] = s.rx;
+        xyz[1] = s.ry;
+        xyz[2] = s.rz;
     }
-    if (ncells) *ncells = c->ncells;
+    if (ncells) *ncells = s.ncells;
+    if (first_slot) *first_slot = s.poff;
+    if (slots) *slots = s.cap;
     return SPH_OK;
+}
+
+sph_status sph_batch_info(sph_ctx *c, uint32_t *scenes, uint32_t *pool, uint32_t *free_slots, uint32_t *active_slots)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    SPH_TRY(ensure_layout(c));
+    uint32_t used = 0;
+    uint64_t taken = 0;
+    for (const SceneRec &r : c->scenes)
+        if (r.used) {
+            ++used;
+            taken += r.s.cap;
+        }
+    if (scenes) *scenes = used;
+    if (pool) *pool = c->pool;
+    if (free_slots) *free_slots = (uint32_t) (c->pool - taken);
+    if (active_slots) *active_slots = c->nslots;
+    return SPH_OK;
+}
+
+sph_status sph_set_graphs(sph_ctx *c, int enable)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->use_graphs = enable != 0;
+    return SPH_OK;
+}
+
+sph_status sph_variant_shape(int variant, int *threads, int *unroll)
+{
+    if (variant < 0 || variant >= sph::kNumVariants)
+        return fail(SPH_ERR_INVALID, "sph_variant_shape: no variant " + std::to_string(variant));
+    if (threads) *threads = sph::kNeighbourVariants[variant].threads;
+    if (unroll) *unroll = sph::kNeighbourVariants[variant].unroll;
+    return SPH_OK;
+}
+
+sph_status sph_set_variant(sph_ctx *c, int variant, int lists)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    if (variant < 0 || variant >= sph::kNumVariants)
+        return fail(SPH_ERR_INVALID, "sph_set_variant: no variant " + std::to_string(variant));
+    if (lists && c->kmax == 0)
+        return fail(SPH_ERR_STATE, "sph_set_variant: the context was created without neighbour lists");
+    SPH_TRY(between_steps(c, "sph_set_variant"));
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    c->variant = variant;
+    c->use_lists = lists != 0;
+    drop_graphs(c);
+    return SPH_OK;
+}
+
+sph_status sph_get_variant(const sph_ctx *c, int *variant, int *lists)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    if (variant) *variant = c->variant;
+    if (lists) *lists = c->use_lists ? 1 : 0;
+    return SPH_OK;
+}
+
+sph_status sph_autotune(sph_ctx *c, uint32_t steps, float *ms_out)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    if (steps == 0) return fail(SPH_ERR_INVALID, "sph_autotune: steps must be > 0");
+    SPH_TRY(between_steps(c, "sph_autotune"));
+    SPH_TRY(check_runnable(c, "sph_autotune"));
+    if (ms_out)
+        for (int i = 0; i < 2 * sph::kNumVariants; ++i) ms_out[i] = -1.0f;
+    const uint32_t saved_flags = c->flags;
+    const long saved_fault = c->fault_at_step;
+    c->flags &= ~(uint32_t) SPH_FLAG_PROFILE;
+    c->fault_at_step = -1;
+    Snapshot snap{c};
+    sph_status st = snap.save();
+    int best_v = c->variant;
+    bool best_l = c->use_lists;
+    float best = -1.0f;
+    for (int lists = 0; lists < 2 && !st; ++lists) {
+        if (lists && c->kmax == 0) break;
+        for (int v = 0; v < sph::kNumVariants && !st; ++v) {
+            st = snap.restore();
+            if (st) break;
+            c->variant = v;
+            c->use_lists = lists != 0;
+            drop_graphs(c);
+            st = run_step(c);   // warm-up (and graph capture)
+            if (!st) st = run_step(c);
+            float ms = 0.0f;
+            if (!st && hipEventRecord(c->span[0], c->stream) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventRecord");
+            for (uint32_t s = 0; s < steps && !st; ++s) st = run_step(c);
+            if (!st && hipEventRecord(c->span[1], c->stream) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventRecord");
+            if (!st && hipEventSynchronize(c->span[1]) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventSynchronize");
+            if (!st && hipEventElapsedTime(&ms, c->span[0], c->span[1]) != hipSuccess)
+                st = fail(SPH_ERR_HIP, "hipEventElapsedTime");
+            if (st) break;
+            ms /= (float) steps;
+            if (ms_out) ms_out[2 * v + lists] = ms;
+            if (best < 0.0f || ms < best) {
+                best = ms;
+                best_v = v;
+                best_l = lists != 0;
+            }
+        }
+    }
+    sph_status st2 = snap.mem ? snap.restore() : SPH_OK;
+    c->flags = saved_flags;
+    c->fault_at_step = saved_fault;
+    c->variant = best_v;
+    c->use_lists = best_l;
+    drop_graphs(c);
+    return st ? st : st2;
 }
 
 sph_status sph_device_name(char *buf, size_t len)
@@ -743,9 +1439,8 @@ sph_status sph_sync(sph_ctx *c)
 sph_status sph_stage_times(sph_ctx *c, float *ms, uint32_t *launches, int reset)
 {
     if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (!(c->prm.flags & SPH_FLAG_PROFILE)) return fail(SPH_ERR_STATE, "sph_stage_times: needs SPH_FLAG_PROFILE");
-    sph_status st = drain_profile(c);
-    if (st) return st;
+    if (!(c->flags & SPH_FLAG_PROFILE)) return fail(SPH_ERR_STATE, "sph_stage_times: needs SPH_FLAG_PROFILE");
+    SPH_TRY(drain_profile(c));
     for (int i = 0; i < SPH_STAGE_COUNT; ++i) {
         if (ms) ms[i] = c->stage_ms[i];
         if (launches) launches[i] = c->stage_launches[i];

@@ -37,16 +37,60 @@ struct Consts {
 };
 
 // One scene: a box with its own uniform grid, a segment of particle slots and a
-// segment of global cell ids. A single-scene context has one of these.
+// segment of global cell ids. The segment of cells ends with one extra "dead"
+// cell (coff + ncells) that unused slots sort into, so that after every sort a
+// scene's live particles are the first n slots of its segment. The kernels read
+// scenes from a table in device memory, so live counts can change on the device
+// and a captured step stays valid. A free range of slots is a scene with no cells
+// and no particles.
 struct Scene {
     int rx, ry, rz;        // grid resolution (CGrid), cells x + y*rx + z*rx*ry
+    uint32_t ncells;       // rx * ry * rz
     uint32_t coff;         // first global cell id of the scene
     uint32_t poff;         // first particle slot of the scene
-    uint32_t n;            // live particles (slots poff .. poff+n-1)
+    uint32_t cap;          // slots of the scene
+    uint32_t n;            // live particles (slots poff .. poff+n-1 after a sort)
     float bx, by, bz;      // box extents (walls at +-b/2)
     float gx, gy, gz;      // gravity
     float dt;
+    uint32_t pad;
 };
+static_assert(sizeof(Scene) == 64, "Scene is read as 16 dwords");
+
+// Slots are grouped in runs of kGroupSlots; a scene's segment starts on a group
+// boundary, so every workgroup of at most kGroupSlots threads lies in one scene.
+constexpr uint32_t kGroupSlots = 256;
+
+// The scenes of a launch as the kernels see them: slot i belongs to scene
+// group_scene[i / kGroupSlots] (group_scene == nullptr: every slot belongs to
+// scene 0); slots at and above nslots are not touched.
+struct SceneTable {
+    const Scene *scenes;
+    const uint32_t *group_scene;
+    uint32_t nslots;
+};
+
+// A particle source that runs on the device at the start of every step and adds
+// particles to one scene (ids n, n+1, ... as CBaseParticleSimulator::addParticle
+// numbers them, src/CBaseParticleSimulator.cpp:67-74).
+enum EmitterKind : uint32_t {
+    kEmitFountain = 0,  // the reference fountain: 7 particles per step at the floor centre
+    kEmitDisc = 1,      // `rate` particles per step uniformly on a floor disc, seeded
+};
+
+struct Emitter {
+    uint32_t scene;        // table index of the scene
+    uint32_t kind;         // EmitterKind
+    uint32_t rate;         // particles per step (disc)
+    uint32_t limit;        // particle budget (fountain: m_maxParticlesCount; disc: stop at limit)
+    float cx, cz;          // disc centre on the floor
+    float radius;          // disc radius
+    float speed;           // mean upward speed (disc)
+    float spread;          // relative speed jitter: v = speed * (1 + spread * (2u - 1))
+    uint32_t seed_lo, seed_hi;
+    uint32_t pad[5];
+};
+static_assert(sizeof(Emitter) == 64, "Emitter is 64 bytes");
 
 // Computes the constants on the host with the reference's expressions.
 Consts make_consts();

@@ -28,7 +28,8 @@ struct DensityFields {
 };
 
 // Accepted neighbours of every particle except itself, in canonical order,
-// stored neighbour-major: idx[k * stride + i] is the k-th neighbour of slot i.
+// stored neighbour-major: idx[k * stride + i] is the k-th neighbour of slot i,
+// as a slot relative to the first slot of i's scene.
 // cnt[i] is the true count; particles with cnt > kmax have no usable list and
 // the force pass re-walks their cells.
 struct NeighbourList {
@@ -48,8 +49,23 @@ constexpr int kScanTile = 4096;  // elements per 1024-thread scan block
 inline uint32_t sort_blocks(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
 inline uint32_t scan_blocks(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 
-// Cell key of every live particle (hash of src/CCPUParticleSimulator.cpp:46-69).
-void launch_hash(const ParticleSoA &p, uint32_t *key, const Scene &s, double h, hipStream_t st);
+// The tunable shape of the two neighbour kernels: workgroup size and candidates
+// tested per lane per trip. Results do not depend on the variant.
+struct NeighbourVariant {
+    int threads;
+    int unroll;
+};
+constexpr int kNumVariants = 6;
+extern const NeighbourVariant kNeighbourVariants[kNumVariants];
+
+// Emitters (one workgroup each) add particles at the start of a step and update
+// the live counts in the device scene table.
+void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleSoA &p, uint32_t *key, double h,
+                 hipStream_t st);
+
+// Cell key of every slot below t.nslots: the hash of src/CCPUParticleSimulator.cpp:46-69
+// for live particles, the scene's dead cell for unused slots.
+void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, double h, hipStream_t st);
 
 // One stable LSD radix pass over 8 bits starting at `shift`. vals_in == nullptr
 // means the identity (val = slot). hist holds kRadixBuckets * sort_blocks(n)
@@ -68,26 +84,26 @@ void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32
 
 // Density pass: density, p/rho^2, 1/rho and the neighbour lists.
 void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
-                    const NeighbourList &nl, const Scene &s, const Consts &k, hipStream_t st);
+                    const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, hipStream_t st);
 
 // Pressure + viscosity + gravity acceleration into acc (the updateForces phase).
 void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
-                   const NeighbourList &nl, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
+                   const NeighbourList &nl, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
                    hipStream_t st);
 
 // acc += wall response (the updateCollisions phase).
-void launch_collide(const ParticleSoA &p, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
+void launch_collide(const ParticleSoA &p, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
                     hipStream_t st);
 
 // Integrate from acc into out and hash the new positions into key_next (the integrate phase).
 void launch_integrate(const ParticleSoA &p, const float *ax, const float *ay, const float *az,
-                      const ParticleSoA &out, uint32_t *key_next, const Scene &s, const Consts &k,
+                      const ParticleSoA &out, uint32_t *key_next, const SceneTable &t, const Consts &k,
                       hipStream_t st);
 
 // Fused forces + walls + integrate + next hash. acc pointers may be null (then not stored).
 void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
                          const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out,
-                         uint32_t *key_next, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
-                         hipStream_t st);
+                         uint32_t *key_next, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
+                         int variant, hipStream_t st);
 
 }  // namespace sph

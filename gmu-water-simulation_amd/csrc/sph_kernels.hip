@@ -12,6 +12,8 @@
 // Data layout: structure-of-arrays FP32 particle buffers in HBM, stable-sorted by
 // cell every step, so that the 3 x-adjacent cells of a grid row are one
 // contiguous run of slots and a particle's 27-cell neighbourhood is 9 runs.
+// Several independent scenes share the buffers: each owns a segment of slots and
+// a segment of cell ids, and one launch of every stage covers all of them.
 
 #include "sph_kernels.h"
 
@@ -94,6 +96,27 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g)
     return x * q + (x < r ? x : r) + l;
 }
 
+// The scene of a workgroup. A workgroup never straddles two scenes (segments
+// start on kGroupSlots boundaries), so the record is read into scalar registers.
+__device__ __forceinline__ Scene block_scene(const SceneTable &t, uint32_t first_slot)
+{
+    const uint32_t sid =
+        t.group_scene ? __builtin_amdgcn_readfirstlane(t.group_scene[first_slot / kGroupSlots]) : 0u;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(t.scenes + sid);
+    uint32_t v[sizeof(Scene) / 4];
+#pragma unroll
+    for (int q = 0; q < (int) (sizeof(Scene) / 4); ++q) v[q] = __builtin_amdgcn_readfirstlane(w[q]);
+    Scene s;
+    __builtin_memcpy(&s, v, sizeof(Scene));
+    return s;
+}
+
+// The key of an unused slot: the scene's dead cell, after all of its grid cells.
+__device__ __forceinline__ uint32_t dead_key(const Scene &s)
+{
+    return s.coff + s.ncells;
+}
+
 __device__ __forceinline__ void cell_xyz(uint32_t key, const Scene &s, int &x, int &y, int &z)
 {
     const uint32_t c = key - s.coff;
@@ -105,13 +128,95 @@ __device__ __forceinline__ void cell_xyz(uint32_t key, const Scene &s, int &x, i
 
 // ---------------------------------------------------------------- grid
 
-__global__ void k_hash(const float *__restrict__ px, const float *__restrict__ py,
-                       const float *__restrict__ pz, uint32_t *__restrict__ key, Scene s, double h)
+__global__ __launch_bounds__(256) void k_hash(const float *__restrict__ px, const float *__restrict__ py,
+                                              const float *__restrict__ pz, uint32_t *__restrict__ key,
+                                              SceneTable t, double h)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
-    const uint32_t g = s.poff + i;
-    key[g] = hash_key(px[g], py[g], pz[g], s, h);
+    const uint32_t first = blockIdx.x * 256u, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    if (i >= t.nslots) return;
+    key[i] = (i - s.poff < s.n) ? hash_key(px[i], py[i], pz[i], s, h) : dead_key(s);
+}
+
+// SplitMix64 output `k` (k >= 1) of the stream that starts at `state`.
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t state, uint64_t k)
+{
+    uint64_t z = state + k * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float unit24(uint64_t x)
+{
+    return (float) (uint32_t) (x >> 40) * 0x1p-24f;
+}
+
+// Emitters, one workgroup each, run before the step's grid phase. The fountain
+// adds the reference's 7 particles (src/CBaseParticleSimulator.cpp:187-210) while
+// n < limit - 7 (unsigned, as the reference compares cl_int with cl_uint); the
+// disc adds min(rate, limit - n) particles. Particle g of a disc emitter draws
+// from its own SplitMix64 stream (state seed + 32*g*golden): 15 tries of a
+// point (2u-1, 2u-1) in the unit square until it falls in the unit disc (else the
+// centre), then v_y = speed * (1 + spread*(2u-1)) from draw 31. Every operation
+// is a single IEEE float operation, so a CPU restatement gives the same bits.
+__global__ __launch_bounds__(256) void k_emit(const Emitter *__restrict__ em, Scene *scenes, ParticleSoA p,
+                                              uint32_t *__restrict__ key, double h)
+{
+    __shared__ uint32_t sh[2];
+    const Emitter e = em[blockIdx.x];
+    Scene *sp = scenes + e.scene;
+    if (threadIdx.x == 0) {
+        const uint32_t n0 = sp->n, cap = sp->cap;
+        uint32_t cnt;
+        if (e.kind == kEmitFountain)
+            cnt = n0 >= e.limit - 7u ? 0u : 7u;
+        else
+            cnt = e.limit > n0 ? min(e.rate, e.limit - n0) : 0u;
+        sh[0] = n0;
+        sh[1] = min(cnt, cap - n0);
+    }
+    __syncthreads();
+    const uint32_t n0 = sh[0], cnt = sh[1];
+    const Scene s = *sp;
+    const float floor_y = -s.by / 2.0f;
+    for (uint32_t q = threadIdx.x; q < cnt; q += blockDim.x) {
+        const uint32_t g = n0 + q, slot = s.poff + g;
+        float x, z, vy;
+        if (e.kind == kEmitFountain) {
+            const float half = kH / 2.0f, quarter = kH / 4.0f;
+            const float xs[7] = {0.0f, -half, half, -quarter, quarter, -quarter, quarter};
+            const float zs[7] = {0.0f, 0.0f, 0.0f, -half, -half, half, half};
+            x = xs[q];
+            z = zs[q];
+            vy = s.by * 3.2f;
+        } else {
+            const uint64_t st = ((uint64_t) e.seed_hi << 32 | e.seed_lo) + 32ull * g * 0x9E3779B97F4A7C15ull;
+            float a = 0.0f, b = 0.0f;
+            for (uint32_t tr = 0; tr < 15; ++tr) {
+                const float u = 2.0f * unit24(splitmix64_at(st, 2 * tr + 1)) - 1.0f;
+                const float w = 2.0f * unit24(splitmix64_at(st, 2 * tr + 2)) - 1.0f;
+                if (u * u + w * w <= 1.0f) {
+                    a = u;
+                    b = w;
+                    break;
+                }
+            }
+            x = e.cx + e.radius * a;
+            z = e.cz + e.radius * b;
+            vy = e.speed * (1.0f + e.spread * (2.0f * unit24(splitmix64_at(st, 31)) - 1.0f));
+        }
+        p.px[slot] = x;
+        p.py[slot] = floor_y;
+        p.pz[slot] = z;
+        p.vx[slot] = 0.0f;
+        p.vy[slot] = vy;
+        p.vz[slot] = 0.0f;
+        p.id[slot] = g;
+        key[slot] = hash_key(x, floor_y, z, s, h);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sp->n = n0 + cnt;
 }
 
 // Digit histogram of one tile, stored digit-major: hist[d * nblocks + block].
@@ -307,15 +412,13 @@ __global__ void k_reorder(ParticleSoA src, ParticleSoA dst, const uint32_t *__re
 // pair arithmetic runs on accepted pairs only (about 1 in 6 candidates) and the
 // summation order of every particle is unchanged.
 
-constexpr int kNeighbourThreads = 256;
 constexpr int kListCap = 16;
-
-constexpr int kUnroll = 4;
 constexpr uint32_t kBufWord3 = 0x00020000;  // descriptor word 3 of a gfx9 raw buffer
 
-// Raw buffer views of the per-particle arrays: loads take 32-bit byte offsets
-// (constant parts fold into the instruction) instead of 64-bit addresses, and
-// reads past the end return 0 instead of faulting.
+// Raw buffer views of one scene's per-particle arrays: loads take 32-bit byte
+// offsets relative to the scene's first slot (constant parts fold into the
+// instruction) instead of 64-bit addresses, and reads outside the scene's
+// segment return 0 instead of faulting.
 struct BufView {
     __amdgpu_buffer_rsrc_t px, py, pz, rho, rec;
 };
@@ -325,14 +428,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *base, uint32_
     return __builtin_amdgcn_make_buffer_rsrc((void *) base, (short) 0, (int) bytes, (int) kBufWord3);
 }
 
-__device__ __forceinline__ BufView buf_view(const ParticleSoA &p, const DensityFields &f, uint32_t slots)
+__device__ __forceinline__ BufView buf_view(const ParticleSoA &p, const DensityFields &f, const Scene &s)
 {
     BufView v;
-    v.px = rsrc(p.px, slots * 4u);
-    v.py = rsrc(p.py, slots * 4u);
-    v.pz = rsrc(p.pz, slots * 4u);
-    v.rho = rsrc(f.rho, slots * 4u);
-    v.rec = rsrc(f.rec, slots * 64u);
+    v.px = rsrc(p.px + s.poff, s.cap * 4u);
+    v.py = rsrc(p.py + s.poff, s.cap * 4u);
+    v.pz = rsrc(p.pz + s.poff, s.cap * 4u);
+    v.rho = rsrc(f.rho + s.poff, s.cap * 4u);
+    v.rec = rsrc(f.rec + 4 * (size_t) s.poff, s.cap * 64u);
     return v;
 }
 
@@ -368,15 +471,16 @@ __device__ __forceinline__ FaceBounds face_bounds(float p, float box, int c, dou
 // a run whose distance lower bound exceeds h (with a 1e-5 relative margin, far
 // above the float rounding of r^2) are left out: every candidate they hold would
 // fail the r^2 <= h^2 test, so the accepted neighbours and their order are
-// unchanged. test(j, valid, xj, yj, zj) sees kUnroll candidates per trip in slot
-// order and appends without branching (writes slot cnt, then adds the accept
-// bit), so a list keeps kUnroll slots of headroom; when any lane's list is that
-// full, flush() drains every lane's list.
-template <typename Test, typename Flush>
+// unchanged. test(j, valid, xj, yj, zj) sees U candidates per trip in slot order
+// (j relative to the scene's first slot) and appends without branching (writes
+// slot cnt, then adds the accept bit), so a list keeps U slots of headroom; when
+// any lane's list is that full, flush() drains every lane's list.
+template <int U, typename Test, typename Flush>
 __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k, const uint32_t *__restrict__ start,
                                                 const BufView &bv, int x, int y, int z, float xi, float yi,
                                                 float zi, int &cnt, Test test, Flush flush)
 {
+    static_assert(U < kListCap, "a list must hold one trip of candidates");
     const FaceBounds bx = face_bounds(xi, s.bx, x, k.h);
     const FaceBounds by = face_bounds(yi, s.by, y, k.h);
     const FaceBounds bz = face_bounds(zi, s.bz, z, k.h);
@@ -390,24 +494,24 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
             const uint32_t row = s.coff + (uint32_t) (yy * s.rx + zz * s.rx * s.ry);
             const int x0 = (x > 0 && lyz + bx.lo2 <= lim) ? x - 1 : x;
             const int x1 = (x + 1 < s.rx && lyz + bx.hi2 <= lim) ? x + 1 : x;
-            j = start[row + x0];
-            jend = start[row + x1 + 1];
+            j = start[row + x0] - s.poff;
+            jend = start[row + x1 + 1] - s.poff;
         }
         while (__any(j < jend)) {
             if (j < jend) {
                 const uint32_t off = j * 4u, rem = jend - j;
-                float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+                float cx[U], cy[U], cz[U];
 #pragma unroll
-                for (int u = 0; u < kUnroll; ++u) {
+                for (int u = 0; u < U; ++u) {
                     cx[u] = ldf(bv.px, off + 4u * u);
                     cy[u] = ldf(bv.py, off + 4u * u);
                     cz[u] = ldf(bv.pz, off + 4u * u);
                 }
 #pragma unroll
-                for (int u = 0; u < kUnroll; ++u) test(j + u, (uint32_t) u < rem, cx[u], cy[u], cz[u]);
-                j += kUnroll;
+                for (int u = 0; u < U; ++u) test(j + u, (uint32_t) u < rem, cx[u], cy[u], cz[u]);
+                j += U;
             }
-            if (__any(cnt > kListCap - kUnroll)) {
+            if (__any(cnt > kListCap - U)) {
                 flush();
                 cnt = 0;
             }
@@ -448,7 +552,7 @@ __device__ __forceinline__ void div3(float a0, float a1, float a2, float b, doub
     q0 = (float) d0;
     q1 = (float) d1;
     q2 = (float) d2;
-    if (__builtin_expect(subnormal_quot(d0) | subnormal_quot(d1) | subnormal_quot(d2), 0)) {
+    if (__builtin_expect((int) subnormal_quot(d0) | (int) subnormal_quot(d1) | (int) subnormal_quot(d2), 0)) {
         q0 = a0 / b;
         q1 = a1 / b;
         q2 = a2 / b;
@@ -458,42 +562,44 @@ __device__ __forceinline__ void div3(float a0, float a1, float a2, float b, doub
 // updateDensityPressure, src/CCPUParticleSimulator.cpp:93-141. Also stores
 // p/rho^2 (double, as src/CCPUParticleSimulator.cpp:181 forms it), 1/rho
 // (double) and, with kLists, the particle's accepted neighbours (minus itself)
-// for the force pass.
-template <bool kLists>
-__global__ __launch_bounds__(kNeighbourThreads) void k_density(ParticleSoA p, const uint32_t *__restrict__ cell,
-                                                               const uint32_t *__restrict__ start, DensityFields f,
-                                                               NeighbourList nl, uint32_t slots, Scene s, Consts k)
+// for the force pass. T threads per workgroup, U candidates per lane per trip.
+template <int T, int U, bool kLists>
+__global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__restrict__ cell,
+                                               const uint32_t *__restrict__ start, DensityFields f,
+                                               NeighbourList nl, SceneTable t, Consts k)
 {
-    __shared__ float lst[kListCap * kNeighbourThreads];
-    const uint32_t li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (li >= s.n) return;
-    const uint32_t i = s.poff + li;
+    static_assert(kGroupSlots % T == 0, "a workgroup must lie in one slot group");
+    __shared__ float lst[kListCap * T];
+    const uint32_t first = xcd_block(blockIdx.x, gridDim.x) * (uint32_t) T, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    const uint32_t li = i - s.poff;
+    if (i >= t.nslots || li >= s.n) return;
     float *my = lst + threadIdx.x;
     uint32_t *nidx = nl.idx + i;
-    const BufView bv = buf_view(p, f, slots);
+    const BufView bv = buf_view(p, f, s);
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     int x, y, z;
     cell_xyz(cell[i], s, x, y, z);
     float density = 0.0f;
     uint32_t nc = 0, nn = 0;
     int cnt = 0;
-    walk_neighbours(
+    walk_neighbours<U>(
         s, k, start, bv, x, y, z, xi, yi, zi, cnt,
         [&](uint32_t j, bool valid, float xj, float yj, float zj) {
             const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
             const float r2 = dx * dx + dyv * dyv + dzv * dzv;
             const bool ok = valid && r2 <= k.hsq_f;
-            my[cnt * kNeighbourThreads] = r2;
+            my[cnt * T] = r2;
             cnt += ok;
             nc += ok;
-            if (kLists && ok && j != i) {
+            if (kLists && ok && j != li) {
                 if (nn < nl.kmax) nidx[(size_t) nn * nl.stride] = j;
                 ++nn;
             }
         },
         [&]() {
             for (int q = 0; q < cnt; ++q) {
-                const double wpoly = k.poly6 * cube_cr(k.hsq - (double) my[q * kNeighbourThreads]);
+                const double wpoly = k.poly6 * cube_cr(k.hsq - (double) my[q * T]);
                 density = (float) ((double) density + wpoly);
             }
         });
@@ -568,7 +674,7 @@ __device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, c
     tx = (float) d0;
     ty = (float) d1;
     tz = (float) d2;
-    if (__builtin_expect(subnormal_quot(d0) | subnormal_quot(d1) | subnormal_quot(d2), 0)) {
+    if (__builtin_expect((int) subnormal_quot(d0) | (int) subnormal_quot(d1) | (int) subnormal_quot(d2), 0)) {
         const float rj = ldf(bv.rho, n.j * 4u);
         tx = t0 / rj;
         ty = t1 / rj;
@@ -579,17 +685,18 @@ __device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, c
     a.vz = a.vz + tz;
 }
 
-// The force sums of particle i followed by the acceleration (f_p + f_v + f_g) / rho
-// (src/CCPUParticleSimulator.cpp:155-193). Neighbours come from the density pass's
-// list when one was kept; otherwise (or on overflow) the cells are walked again
-// in the same order.
+// The force sums of particle i (slot li of its scene) followed by the
+// acceleration (f_p + f_v + f_g) / rho (src/CCPUParticleSimulator.cpp:155-193).
+// Neighbours come from the density pass's list when one was kept; otherwise (or
+// on overflow) the cells are walked again in the same order.
+template <int T, int U>
 __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *__restrict__ cell,
                                           const uint32_t *__restrict__ start, const DensityFields &f,
-                                          const NeighbourList &nl, uint32_t *my, uint32_t i, uint32_t slots,
+                                          const NeighbourList &nl, uint32_t *my, uint32_t i, uint32_t li,
                                           const Scene &s, const Consts &k, float xi, float yi, float zi, float vxi,
                                           float vyi, float vzi, float &ax, float &ay, float &az)
 {
-    const BufView bv = buf_view(p, f, slots);
+    const BufView bv = buf_view(p, f, s);
     const float rhoi = f.rho[i];
     const double ai = f.prr[i];
     ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -610,20 +717,20 @@ __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *
         int x, y, z;
         cell_xyz(cell[i], s, x, y, z);
         int cnt = 0;
-        walk_neighbours(
+        walk_neighbours<U>(
             s, k, start, bv, x, y, z, xi, yi, zi, cnt,
             [&](uint32_t j, bool valid, float xj, float yj, float zj) {
                 const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
                 const float r2 = dx * dx + dyv * dyv + dzv * dzv;
-                my[cnt * kNeighbourThreads] = j;
-                cnt += valid && r2 <= k.hsq_f && j != i;
+                my[cnt * T] = j;
+                cnt += valid && r2 <= k.hsq_f && j != li;
             },
             [&]() {
                 if (cnt == 0) return;
                 PairData nxt = load_pair(bv, my[0]);
                 for (int q = 0; q < cnt; ++q) {
                     const PairData cur = nxt;
-                    if (q + 1 < cnt) nxt = load_pair(bv, my[(q + 1) * kNeighbourThreads]);
+                    if (q + 1 < cnt) nxt = load_pair(bv, my[(q + 1) * T]);
                     force_pair(bv, k, cur, xi, yi, zi, vxi, vyi, vzi, ai, a);
                 }
             });
@@ -650,30 +757,34 @@ __device__ __forceinline__ void integrate_one(float xi, float yi, float zi, floa
     nvz = (nz - zi) / dt;
 }
 
-__global__ __launch_bounds__(kNeighbourThreads) void k_forces(ParticleSoA p, const uint32_t *__restrict__ cell,
-                                                              const uint32_t *__restrict__ start, DensityFields f,
-                                                              NeighbourList nl, uint32_t slots, float *__restrict__ ax,
-                                                              float *__restrict__ ay, float *__restrict__ az,
-                                                              Scene s, Consts k)
+constexpr int kPhaseThreads = 256;   // workgroup of the phase-by-phase kernels
+constexpr int kPhaseUnroll = 4;
+
+__global__ __launch_bounds__(kPhaseThreads) void k_forces(ParticleSoA p, const uint32_t *__restrict__ cell,
+                                                          const uint32_t *__restrict__ start, DensityFields f,
+                                                          NeighbourList nl, float *__restrict__ ax,
+                                                          float *__restrict__ ay, float *__restrict__ az,
+                                                          SceneTable t, Consts k)
 {
-    __shared__ uint32_t lst[kListCap * kNeighbourThreads];
-    const uint32_t li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (li >= s.n) return;
-    const uint32_t i = s.poff + li;
+    __shared__ uint32_t lst[kListCap * kPhaseThreads];
+    const uint32_t first = xcd_block(blockIdx.x, gridDim.x) * (uint32_t) kPhaseThreads, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    const uint32_t li = i - s.poff;
+    if (i >= t.nslots || li >= s.n) return;
     float a0, a1, a2;
-    sph_accel(p, cell, start, f, nl, lst + threadIdx.x, i, slots, s, k, p.px[i], p.py[i], p.pz[i], p.vx[i],
-              p.vy[i], p.vz[i], a0, a1, a2);
+    sph_accel<kPhaseThreads, kPhaseUnroll>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, p.px[i], p.py[i],
+                                           p.pz[i], p.vx[i], p.vy[i], p.vz[i], a0, a1, a2);
     ax[i] = a0;
     ay[i] = a1;
     az[i] = a2;
 }
 
-__global__ void k_collide(ParticleSoA p, float *__restrict__ ax, float *__restrict__ ay, float *__restrict__ az,
-                          Scene s, Consts k)
+__global__ __launch_bounds__(256) void k_collide(ParticleSoA p, float *__restrict__ ax, float *__restrict__ ay,
+                                                 float *__restrict__ az, SceneTable t, Consts k)
 {
-    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= s.n) return;
-    const uint32_t i = s.poff + li;
+    const uint32_t first = blockIdx.x * 256u, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    if (i >= t.nslots || i - s.poff >= s.n) return;
     float a0 = ax[i], a1 = ay[i], a2 = az[i];
     wall_accel(p.px[i], p.py[i], p.pz[i], p.vx[i], p.vy[i], p.vz[i], s, k, a0, a1, a2);
     ax[i] = a0;
@@ -681,13 +792,18 @@ __global__ void k_collide(ParticleSoA p, float *__restrict__ ax, float *__restri
     az[i] = a2;
 }
 
-__global__ void k_integrate(ParticleSoA p, const float *__restrict__ ax, const float *__restrict__ ay,
-                            const float *__restrict__ az, ParticleSoA out, uint32_t *__restrict__ key_next,
-                            Scene s, Consts k)
+__global__ __launch_bounds__(256) void k_integrate(ParticleSoA p, const float *__restrict__ ax,
+                                                   const float *__restrict__ ay, const float *__restrict__ az,
+                                                   ParticleSoA out, uint32_t *__restrict__ key_next, SceneTable t,
+                                                   Consts k)
 {
-    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= s.n) return;
-    const uint32_t i = s.poff + li;
+    const uint32_t first = blockIdx.x * 256u, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    if (i >= t.nslots) return;
+    if (i - s.poff >= s.n) {
+        key_next[i] = dead_key(s);
+        return;
+    }
     float nx, ny, nz, nvx, nvy, nvz;
     integrate_one(p.px[i], p.py[i], p.pz[i], p.vx[i], p.vy[i], p.vz[i], ax[i], ay[i], az[i], s.dt, nx, ny, nz,
                   nvx, nvy, nvz);
@@ -697,21 +813,29 @@ __global__ void k_integrate(ParticleSoA p, const float *__restrict__ ax, const f
 }
 
 // Forces, walls and integration of one particle in one pass, plus the cell key of
-// its new position for the next step's sort.
-template <bool kStoreAcc>
-__global__ __launch_bounds__(kNeighbourThreads) void k_forces_fused(
-    ParticleSoA p, const uint32_t *__restrict__ cell, const uint32_t *__restrict__ start, DensityFields f,
-    NeighbourList nl, uint32_t slots, ParticleSoA out, uint32_t *__restrict__ key_next, float *__restrict__ ax,
-    float *__restrict__ ay, float *__restrict__ az, Scene s, Consts k)
+// its new position for the next step's sort (the dead cell for unused slots).
+template <int T, int U, bool kStoreAcc>
+__global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_t *__restrict__ cell,
+                                                    const uint32_t *__restrict__ start, DensityFields f,
+                                                    NeighbourList nl, ParticleSoA out,
+                                                    uint32_t *__restrict__ key_next, float *__restrict__ ax,
+                                                    float *__restrict__ ay, float *__restrict__ az, SceneTable t,
+                                                    Consts k)
 {
-    __shared__ uint32_t lst[kListCap * kNeighbourThreads];
-    const uint32_t li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (li >= s.n) return;
-    const uint32_t i = s.poff + li;
+    static_assert(kGroupSlots % T == 0, "a workgroup must lie in one slot group");
+    __shared__ uint32_t lst[kListCap * T];
+    const uint32_t first = xcd_block(blockIdx.x, gridDim.x) * (uint32_t) T, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    const uint32_t li = i - s.poff;
+    if (i >= t.nslots) return;
+    if (li >= s.n) {
+        key_next[i] = dead_key(s);
+        return;
+    }
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     const float vxi = p.vx[i], vyi = p.vy[i], vzi = p.vz[i];
     float a0, a1, a2;
-    sph_accel(p, cell, start, f, nl, lst + threadIdx.x, i, slots, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1, a2);
+    sph_accel<T, U>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1, a2);
     wall_accel(xi, yi, zi, vxi, vyi, vzi, s, k, a0, a1, a2);
     if (kStoreAcc) {
         ax[i] = a0;
@@ -727,14 +851,50 @@ __global__ __launch_bounds__(kNeighbourThreads) void k_forces_fused(
 
 inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 
+// The compiled shapes of the two neighbour kernels, indexed by variant.
+template <int T, int U>
+struct NeighbourKernels {
+    static void density(const dim3 &g, hipStream_t st, bool lists, const ParticleSoA &p, const uint32_t *cell,
+                        const uint32_t *start, const DensityFields &f, const NeighbourList &nl, const SceneTable &t,
+                        const Consts &k)
+    {
+        if (lists)
+            hipLaunchKernelGGL((k_density<T, U, true>), g, dim3(T), 0, st, p, cell, start, f, nl, t, k);
+        else
+            hipLaunchKernelGGL((k_density<T, U, false>), g, dim3(T), 0, st, p, cell, start, f, nl, t, k);
+    }
+    static void fused(const dim3 &g, hipStream_t st, const ParticleSoA &p, const uint32_t *cell,
+                      const uint32_t *start, const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out,
+                      uint32_t *key_next, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k)
+    {
+        if (ax)
+            hipLaunchKernelGGL((k_forces_fused<T, U, true>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
+                               key_next, ax, ay, az, t, k);
+        else
+            hipLaunchKernelGGL((k_forces_fused<T, U, false>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
+                               key_next, ax, ay, az, t, k);
+    }
+};
+
 }  // namespace
+
+const NeighbourVariant kNeighbourVariants[kNumVariants] = {
+    {256, 4}, {128, 4}, {64, 4}, {256, 2}, {128, 2}, {256, 8},
+};
 
 // ---------------------------------------------------------------- launchers
 
-void launch_hash(const ParticleSoA &p, uint32_t *key, const Scene &s, double h, hipStream_t st)
+void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleSoA &p, uint32_t *key, double h,
+                 hipStream_t st)
 {
-    if (s.n == 0) return;
-    hipLaunchKernelGGL(k_hash, dim3(blocks_for(s.n, 256)), dim3(256), 0, st, p.px, p.py, p.pz, key, s, h);
+    if (nem == 0) return;
+    hipLaunchKernelGGL(k_emit, dim3(nem), dim3(256), 0, st, em, scenes, p, key, h);
+}
+
+void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, double h, hipStream_t st)
+{
+    if (t.nslots == 0) return;
+    hipLaunchKernelGGL(k_hash, dim3(blocks_for(t.nslots, 256)), dim3(256), 0, st, p.px, p.py, p.pz, key, t, h);
 }
 
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out,
@@ -770,52 +930,57 @@ void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32
 }
 
 void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
-                    const NeighbourList &nl, const Scene &s, const Consts &k, hipStream_t st)
+                    const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, hipStream_t st)
 {
-    if (s.n == 0) return;
-    const dim3 grid(blocks_for(s.n, kNeighbourThreads)), block(kNeighbourThreads);
-    if (nl.kmax)
-        hipLaunchKernelGGL(k_density<true>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, s, k);
-    else
-        hipLaunchKernelGGL(k_density<false>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, s, k);
+    if (t.nslots == 0) return;
+    const bool lists = nl.kmax != 0;
+    switch (variant) {
+    case 1: NeighbourKernels<128, 4>::density(dim3(blocks_for(t.nslots, 128)), st, lists, p, cell, start, f, nl, t, k); break;
+    case 2: NeighbourKernels<64, 4>::density(dim3(blocks_for(t.nslots, 64)), st, lists, p, cell, start, f, nl, t, k); break;
+    case 3: NeighbourKernels<256, 2>::density(dim3(blocks_for(t.nslots, 256)), st, lists, p, cell, start, f, nl, t, k); break;
+    case 4: NeighbourKernels<128, 2>::density(dim3(blocks_for(t.nslots, 128)), st, lists, p, cell, start, f, nl, t, k); break;
+    case 5: NeighbourKernels<256, 8>::density(dim3(blocks_for(t.nslots, 256)), st, lists, p, cell, start, f, nl, t, k); break;
+    default: NeighbourKernels<256, 4>::density(dim3(blocks_for(t.nslots, 256)), st, lists, p, cell, start, f, nl, t, k); break;
+    }
 }
 
 void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
-                   const NeighbourList &nl, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
+                   const NeighbourList &nl, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
                    hipStream_t st)
 {
-    if (s.n == 0) return;
-    hipLaunchKernelGGL(k_forces, dim3(blocks_for(s.n, kNeighbourThreads)), dim3(kNeighbourThreads), 0, st, p, cell,
-                       start, f, nl, nl.stride, ax, ay, az, s, k);
+    if (t.nslots == 0) return;
+    hipLaunchKernelGGL(k_forces, dim3(blocks_for(t.nslots, kPhaseThreads)), dim3(kPhaseThreads), 0, st, p, cell,
+                       start, f, nl, ax, ay, az, t, k);
 }
 
-void launch_collide(const ParticleSoA &p, float *ax, float *ay, float *az, const Scene &s, const Consts &k,
+void launch_collide(const ParticleSoA &p, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
                     hipStream_t st)
 {
-    if (s.n == 0) return;
-    hipLaunchKernelGGL(k_collide, dim3(blocks_for(s.n, 256)), dim3(256), 0, st, p, ax, ay, az, s, k);
+    if (t.nslots == 0) return;
+    hipLaunchKernelGGL(k_collide, dim3(blocks_for(t.nslots, 256)), dim3(256), 0, st, p, ax, ay, az, t, k);
 }
 
 void launch_integrate(const ParticleSoA &p, const float *ax, const float *ay, const float *az, const ParticleSoA &out,
-                      uint32_t *key_next, const Scene &s, const Consts &k, hipStream_t st)
+                      uint32_t *key_next, const SceneTable &t, const Consts &k, hipStream_t st)
 {
-    if (s.n == 0) return;
-    hipLaunchKernelGGL(k_integrate, dim3(blocks_for(s.n, 256)), dim3(256), 0, st, p, ax, ay, az, out, key_next, s,
-                       k);
+    if (t.nslots == 0) return;
+    hipLaunchKernelGGL(k_integrate, dim3(blocks_for(t.nslots, 256)), dim3(256), 0, st, p, ax, ay, az, out, key_next,
+                       t, k);
 }
 
 void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
                          const NeighbourList &nl, const ParticleSoA &out, uint32_t *key_next, float *ax, float *ay,
-                         float *az, const Scene &s, const Consts &k, hipStream_t st)
+                         float *az, const SceneTable &t, const Consts &k, int variant, hipStream_t st)
 {
-    if (s.n == 0) return;
-    const dim3 grid(blocks_for(s.n, kNeighbourThreads)), block(kNeighbourThreads);
-    if (ax)
-        hipLaunchKernelGGL(k_forces_fused<true>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, out, key_next,
-                           ax, ay, az, s, k);
-    else
-        hipLaunchKernelGGL(k_forces_fused<false>, grid, block, 0, st, p, cell, start, f, nl, nl.stride, out,
-                           key_next, ax, ay, az, s, k);
+    if (t.nslots == 0) return;
+    switch (variant) {
+    case 1: NeighbourKernels<128, 4>::fused(dim3(blocks_for(t.nslots, 128)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    case 2: NeighbourKernels<64, 4>::fused(dim3(blocks_for(t.nslots, 64)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    case 3: NeighbourKernels<256, 2>::fused(dim3(blocks_for(t.nslots, 256)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    case 4: NeighbourKernels<128, 2>::fused(dim3(blocks_for(t.nslots, 128)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    case 5: NeighbourKernels<256, 8>::fused(dim3(blocks_for(t.nslots, 256)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    default: NeighbourKernels<256, 4>::fused(dim3(blocks_for(t.nslots, 256)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    }
 }
 
 }  // namespace sph

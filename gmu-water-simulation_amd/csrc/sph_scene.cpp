@@ -52,22 +52,22 @@ sph_status sph_scene_dam_break(float box, float *pos3, uint32_t cap, uint32_t *n
     return SPH_OK;
 }
 
-sph_status sph_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float box[3], float jitter_h,
-                             uint64_t seed, float *pos3)
+sph_status sph_scene_block(const float origin[3], uint32_t nx, uint32_t ny, uint32_t nz, uint32_t count,
+                           float jitter_h, uint64_t seed, float *pos3)
 {
-    if (!box || !pos3) return SPH_ERR_INVALID;
+    if (!origin || (!pos3 && count > 0)) return SPH_ERR_INVALID;
+    if ((uint64_t) count > (uint64_t) nx * ny * nz) return SPH_ERR_INVALID;
     const double half = (double) (sph::kH / 2.0f);
-    const float ox = -box[0] / 2.0f, oy = -box[1] / 2.0f, oz = -box[2] / 2.0f;
     float cx = 0, cy = 0, cz;
     size_t id = 0;
-    for (uint32_t j = 0; j < ny; ++j) {
+    for (uint32_t j = 0; j < ny && id < count; ++j) {
         cx = 0;
-        for (uint32_t i = 0; i < nx; ++i) {
+        for (uint32_t i = 0; i < nx && id < count; ++i) {
             cz = 0;
-            for (uint32_t k = 0; k < nz; ++k) {
-                pos3[3 * id + 0] = cx + ox;
-                pos3[3 * id + 1] = cy + oy;
-                pos3[3 * id + 2] = cz + oz;
+            for (uint32_t k = 0; k < nz && id < count; ++k) {
+                pos3[3 * id + 0] = cx + origin[0];
+                pos3[3 * id + 1] = cy + origin[1];
+                pos3[3 * id + 2] = cz + origin[2];
                 ++id;
                 cz = (float) ((double) cz + half);
             }
@@ -83,6 +83,16 @@ sph_status sph_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float 
             pos3[3 * p + a] = pos3[3 * p + a] + j;
         }
     return SPH_OK;
+}
+
+sph_status sph_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float box[3], float jitter_h,
+                             uint64_t seed, float *pos3)
+{
+    if (!box || !pos3) return SPH_ERR_INVALID;
+    const float origin[3] = {-box[0] / 2.0f, -box[1] / 2.0f, -box[2] / 2.0f};
+    const uint64_t count = (uint64_t) nx * ny * nz;
+    if (count > 0xFFFFFFFFull) return SPH_ERR_INVALID;
+    return sph_scene_block(origin, nx, ny, nz, (uint32_t) count, jitter_h, seed, pos3);
 }
 
 }  // extern "C"

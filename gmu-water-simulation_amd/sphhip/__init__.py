@@ -4,3 +4,4 @@ from . import _lib  # noqa: F401
 from ._lib import SphError, load, LIB_PATH  # noqa: F401
 from .simulator import HIPGridParticleSimulator, DAM_BREAK, FOUNTAIN  # noqa: F401
 from . import scenes  # noqa: F401
+from .batch import SceneBatch, variant_shape  # noqa: F401

@@ -20,6 +20,9 @@ NEIGHBOUR_LISTS_OFF = 0xFFFFFFFF
 (DEBUG_CELL_PRESORT, DEBUG_PERM, DEBUG_CELL, DEBUG_CELL_START, DEBUG_NEIGHBOURS, DEBUG_DENSITY,
  DEBUG_PRESSURE, DEBUG_ACCEL, DEBUG_IDS) = range(9)
 STAGES = ["hash", "sort", "cells", "reorder", "density", "forces", "collisions", "integrate"]
+EMIT_FOUNTAIN, EMIT_DISC = 0, 1
+NUM_VARIANTS = 6
+GROUP_SLOTS = 256
 
 # Every entry point of include/sph_hip.h with its ctypes signature.
 _vp = C.c_void_p
@@ -45,12 +48,45 @@ SIGNATURES = {
     "sph_last_error": (C.c_char_p, []),
     "sph_scene_dam_break": (C.c_int, [_f, _vp, _u32, C.POINTER(_u32)]),
     "sph_scene_lattice": (C.c_int, [_u32, _u32, _u32, _vp, _f, C.c_uint64, _vp]),
+    "sph_scene_block": (C.c_int, [_vp, _u32, _u32, _u32, _u32, _f, C.c_uint64, _vp]),
+    "sph_default_scene": (None, [_vp, _f, _u32]),
+    "sph_batch_create": (C.c_int, [_vp, _vp, _u32, C.POINTER(_vp)]),
+    "sph_scene_add": (C.c_int, [_vp, _vp, C.POINTER(_u32)]),
+    "sph_scene_remove": (C.c_int, [_vp, _u32]),
+    "sph_scene_upload": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _u32]),
+    "sph_scene_append": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _u32]),
+    "sph_scene_download": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _u32, C.c_int]),
+    "sph_scene_set_gravity": (C.c_int, [_vp, _u32, _f, _f, _f]),
+    "sph_scene_count": (C.c_int, [_vp, _u32, C.POINTER(_u32)]),
+    "sph_scene_debug": (C.c_int, [_vp, _u32, C.c_int, _vp, C.c_size_t]),
+    "sph_scene_grid": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp]),
+    "sph_batch_info": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "sph_scene_set_emitter": (C.c_int, [_vp, _u32, _vp]),
+    "sph_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "sph_set_variant": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "sph_get_variant": (C.c_int, [_vp, _vp, _vp]),
+    "sph_variant_shape": (C.c_int, [C.c_int, _vp, _vp]),
+    "sph_autotune": (C.c_int, [_vp, _u32, _vp]),
 }
 
 
 class sph_params(C.Structure):
     _fields_ = [("box", C.c_float * 3), ("gravity", C.c_float * 3), ("dt", C.c_float),
                 ("capacity", C.c_uint32), ("flags", C.c_uint32), ("max_neighbours", C.c_uint32)]
+
+
+class sph_scene_desc(C.Structure):
+    _fields_ = [("box", C.c_float * 3), ("gravity", C.c_float * 3), ("dt", C.c_float), ("capacity", C.c_uint32)]
+
+
+class sph_batch_params(C.Structure):
+    _fields_ = [("pool", C.c_uint32), ("max_scenes", C.c_uint32), ("flags", C.c_uint32),
+                ("max_neighbours", C.c_uint32)]
+
+
+class sph_emitter(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("rate", C.c_uint32), ("limit", C.c_uint32), ("centre", C.c_float * 2),
+                ("radius", C.c_float), ("speed", C.c_float), ("spread", C.c_float), ("seed", C.c_uint64)]
 
 
 class SphError(RuntimeError):

@@ -24,7 +24,8 @@ H = np.float32(0.0457)
 class HIPGridParticleSimulator:
     """Drop-in for CGPUParticleSimulator (src/CGPUParticleSimulator.cpp)."""
 
-    def __init__(self, box_size, scenario=DAM_BREAK, capacity=None, flags=0, fused=True, max_neighbours=0):
+    def __init__(self, box_size, scenario=DAM_BREAK, capacity=None, flags=0, fused=True, max_neighbours=0,
+                 device_emitter=False):
         lib = _lib.load()
         self._lib = lib
         box = np.broadcast_to(np.asarray(box_size, np.float32), (3,)).astype(np.float32)
@@ -34,6 +35,9 @@ class HIPGridParticleSimulator:
         self.fused = fused
         self.flags = int(flags)
         self.max_neighbours = int(max_neighbours)
+        # FOUNTAIN: add the 7 particles of generateParticles on the device
+        # (sph_scene_set_emitter) instead of appending them from the host
+        self.device_emitter = bool(device_emitter)
         self._ctx = None
         self._capacity = capacity
         self.m_particlesCount = 0
@@ -94,6 +98,11 @@ class HIPGridParticleSimulator:
             self._create(cap)
             self.m_maxParticlesCount = self._calculated_count()
             self.m_particlesCount = 0
+            if self.device_emitter:
+                e = _lib.sph_emitter()
+                e.kind = _lib.EMIT_FOUNTAIN
+                e.limit = self.m_maxParticlesCount
+                _lib.check(self._lib.sph_scene_set_emitter(self._ctx, 0, C.byref(e)))
 
     def setGravityVector(self, g):
         self.gravity = np.asarray(g, np.float32).reshape(3)
@@ -112,8 +121,15 @@ class HIPGridParticleSimulator:
         _lib.check(self._lib.sph_device_name(buf, 256))
         return buf.value.decode()
 
-    def getParticlesCount(self):
+    def _refresh_count(self):
+        if self.device_emitter and self._ctx is not None:
+            n = C.c_uint32(0)
+            _lib.check(self._lib.sph_count(self._ctx, C.byref(n)))
+            self.m_particlesCount = n.value
         return self.m_particlesCount
+
+    def getParticlesCount(self):
+        return self._refresh_count()
 
     def getMaxParticlesCount(self):
         return self.m_maxParticlesCount
@@ -121,7 +137,7 @@ class HIPGridParticleSimulator:
     def generateParticles(self):
         """The reference fountain: 7 particles per step at the floor centre
         (src/CBaseParticleSimulator.cpp:187-210), appended on the device."""
-        if self.m_scenario != FOUNTAIN:
+        if self.m_scenario != FOUNTAIN or self.device_emitter:
             return
         per = 7
         if self.m_particlesCount >= (self.m_maxParticlesCount - per) % (1 << 32):
@@ -138,7 +154,7 @@ class HIPGridParticleSimulator:
     def step(self):
         """CBaseParticleSimulator::step (src/CBaseParticleSimulator.cpp:116-144)."""
         self.generateParticles()
-        if self.m_particlesCount == 0:
+        if self.m_particlesCount == 0 and not self.device_emitter:
             return
         if self.fused:
             _lib.check(self._lib.sph_step(self._ctx, 1, None))
@@ -201,7 +217,7 @@ class HIPGridParticleSimulator:
         self.m_particlesCount += len(pos)
 
     def download(self, by_id=True):
-        n = self.m_particlesCount
+        n = self._refresh_count()
         pos = np.zeros((n, 3), np.float32)
         vel = np.zeros((n, 3), np.float32)
         ids = np.zeros(n, np.uint32)
@@ -215,7 +231,7 @@ class HIPGridParticleSimulator:
         return tuple(int(v) for v in xyz), nc.value
 
     def debug(self, kind):
-        n = self.m_particlesCount
+        n = self._refresh_count()
         if kind == _lib.DEBUG_CELL_START:
             out = np.zeros(self.grid_dims()[1] + 1, np.uint32)
         elif kind in (_lib.DEBUG_DENSITY, _lib.DEBUG_PRESSURE):

@@ -178,6 +178,119 @@ sph_status sph_scene_dam_break(float box, float *pos3, uint32_t cap, uint32_t *n
 sph_status sph_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float box[3],
                              float jitter_h, uint64_t seed, float *pos3);
 
+/* Block of a benchmark lattice (no reference counterpart): the first `count` sites
+ * (count <= nx*ny*nz) of the lattice of sph_scene_lattice placed with its first site at
+ * `origin` instead of -box/2, with the same jitter draws. sph_scene_lattice(nx, ny, nz,
+ * box, ...) is the block at origin -box/2 with count nx*ny*nz. */
+sph_status sph_scene_block(const float origin[3], uint32_t nx, uint32_t ny, uint32_t nz, uint32_t count,
+                           float jitter_h, uint64_t seed, float *pos3);
+
+/* ------------------------------------------------------------------------------
+ * Batched scenes. No counterpart in the reference, which advances one scene per
+ * CBaseParticleSimulator (src/CBaseParticleSimulator.cpp:116-144): a batch context
+ * advances many independent scenes with one launch per stage. Scenes share a pool of
+ * particle slots (each scene a segment starting on a multiple of 256) and a global
+ * cell-id space (each scene its own grid plus one cell for unused slots). A context
+ * made by sph_create is a batch of one scene, id 0; the single-scene entry points
+ * above act on scene 0 (sph_step and sph_phase always advance every scene).
+ * Every scene computes exactly what it computes alone in a context of its own.
+ * ------------------------------------------------------------------------------ */
+
+typedef struct {
+    float box[3];       /* walls at +-box/2 */
+    float gravity[3];
+    float dt;
+    uint32_t capacity;  /* particle slots reserved for the scene */
+} sph_scene_desc;
+
+typedef struct {
+    uint32_t pool;           /* slots shared by all scenes; 0 = the scenes' capacities, each rounded up to 256 */
+    uint32_t max_scenes;     /* scene ids 0..max_scenes-1; 0 = nscenes */
+    uint32_t flags;          /* SPH_FLAG_* */
+    uint32_t max_neighbours; /* as in sph_params */
+} sph_batch_params;
+
+/* Reference defaults for a cubic box of edge `box` (as sph_default_params). */
+void sph_default_scene(sph_scene_desc *d, float box, uint32_t capacity);
+
+/* Creates a context with the given scenes (ids 0..nscenes-1, placed in order).
+ * nscenes may be 0 (scenes are then added with sph_scene_add). */
+sph_status sph_batch_create(const sph_batch_params *bp, const sph_scene_desc *scenes, uint32_t nscenes,
+                            sph_ctx **out);
+
+/* Adds a scene in the first free range of the pool that holds it (SPH_ERR_CAPACITY if
+ * none does) and returns its id. Only between steps. */
+sph_status sph_scene_add(sph_ctx *ctx, const sph_scene_desc *desc, uint32_t *scene);
+
+/* Frees a scene's slots and id. Only between steps. */
+sph_status sph_scene_remove(sph_ctx *ctx, uint32_t scene);
+
+/* sph_upload / sph_append / sph_download / sph_set_gravity / sph_count / sph_debug for
+ * one scene. sph_scene_debug returns the scene's own slots and cells: slot indices
+ * (SPH_DEBUG_PERM, SPH_DEBUG_CELL_START) are relative to the scene's first slot and
+ * cell ids (SPH_DEBUG_CELL_PRESORT, SPH_DEBUG_CELL) to its first cell. */
+sph_status sph_scene_upload(sph_ctx *ctx, uint32_t scene, const float *pos3, const float *vel3,
+                            const uint32_t *ids, uint32_t n);
+sph_status sph_scene_append(sph_ctx *ctx, uint32_t scene, const float *pos3, const float *vel3,
+                            const uint32_t *ids, uint32_t n);
+sph_status sph_scene_download(sph_ctx *ctx, uint32_t scene, float *pos3, float *vel3, uint32_t *ids,
+                              uint32_t n, int by_id);
+sph_status sph_scene_set_gravity(sph_ctx *ctx, uint32_t scene, float gx, float gy, float gz);
+sph_status sph_scene_count(sph_ctx *ctx, uint32_t scene, uint32_t *n);
+sph_status sph_scene_debug(sph_ctx *ctx, uint32_t scene, int kind, void *out, size_t bytes);
+
+/* Grid of one scene (as sph_grid_dims) and its segment of slots. */
+sph_status sph_scene_grid(sph_ctx *ctx, uint32_t scene, int *xyz, uint32_t *ncells, uint32_t *first_slot,
+                          uint32_t *slots);
+
+/* Pool summary: scene ids in use, slots in the pool, free slots, and the slots the
+ * next step covers (sorted and launched over). Any pointer may be NULL. */
+sph_status sph_batch_info(sph_ctx *ctx, uint32_t *scenes, uint32_t *pool, uint32_t *free_slots,
+                          uint32_t *active_slots);
+
+/* Device particle sources. */
+enum {
+    SPH_EMIT_FOUNTAIN = 0, /* the reference fountain: 7 particles per step at the floor centre
+                              while count < limit - 7 (src/CBaseParticleSimulator.cpp:187-210) */
+    SPH_EMIT_DISC = 1      /* `rate` particles per step uniformly on a floor disc until count = limit */
+};
+
+typedef struct {
+    uint32_t kind;      /* SPH_EMIT_* */
+    uint32_t rate;      /* DISC: particles per step */
+    uint32_t limit;     /* FOUNTAIN: m_maxParticlesCount; DISC: final count (at most the capacity) */
+    float centre[2];    /* DISC: x and z of the disc centre (y is the floor, -box_y/2) */
+    float radius;       /* DISC */
+    float speed;        /* DISC: mean upward speed */
+    float spread;       /* DISC: v_y = speed * (1 + spread * (2u - 1)) */
+    uint64_t seed;      /* DISC: SplitMix64 seed; particle g uses the stream at seed + 32*g*0x9E3779B97F4A7C15 */
+} sph_emitter;
+
+/* Replaces: generateParticles + addParticle (src/CBaseParticleSimulator.cpp:67-74, 187-210)
+ * and the full re-upload that follows (src/CGPUParticleSimulator.cpp:61). Attaches an
+ * emitter that runs on the device at the start of every step (before the grid phase),
+ * without host round trips; e == NULL detaches. Particle ids continue from the count. */
+sph_status sph_scene_set_emitter(sph_ctx *ctx, uint32_t scene, const sph_emitter *e);
+
+/* Captures each step of sph_step in a HIP graph and replays it (default off; never with
+ * SPH_FLAG_PROFILE). Results are identical with and without graphs. */
+sph_status sph_set_graphs(sph_ctx *ctx, int enable);
+
+/* Shape of the neighbour kernels: variant 0..SPH_NUM_VARIANTS-1 (workgroup size and
+ * candidates per trip, see sph_variant_shape) and lists != 0 to keep neighbour lists
+ * (only if the context was created with max_neighbours > 0). Results are identical for
+ * every choice. */
+#define SPH_NUM_VARIANTS 6
+sph_status sph_set_variant(sph_ctx *ctx, int variant, int lists);
+sph_status sph_get_variant(const sph_ctx *ctx, int *variant, int *lists);
+sph_status sph_variant_shape(int variant, int *threads, int *unroll);
+
+/* Times `steps` steps of every (variant, lists) choice from the current state, keeps the
+ * fastest and restores the state, so the call changes nothing but the choice. ms_out
+ * (may be NULL) receives 2*SPH_NUM_VARIANTS ms-per-step figures, index 2*variant + lists,
+ * negative where a choice was not run. */
+sph_status sph_autotune(sph_ctx *ctx, uint32_t steps, float *ms_out);
+
 #ifdef __cplusplus
 }
 #endif

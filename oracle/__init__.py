@@ -7,6 +7,6 @@ package, and only as the checker: the HIP product path never calls it.
 """
 from .oracle import (  # noqa: F401
     OracleSim, constants, grid_res, dam_break_count, hash_positions,
-    stable_perm, cell_start, wall_accel, lib_path, build,
+    stable_perm, cell_start, wall_accel, lib_path, build, disc_emission,
     DAM_BREAK, FOUNTAIN, ORDER_REFERENCE, ORDER_CANONICAL,
 )

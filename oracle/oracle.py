@@ -136,6 +136,50 @@ def wall_accel(pos, vel, box):
     return out
 
 
+_GOLDEN = 0x9E3779B97F4A7C15
+_M64 = (1 << 64) - 1
+
+
+def _splitmix64_at(state, k):
+    """Output k (k >= 1) of the SplitMix64 stream that starts at `state`."""
+    z = (state + k * _GOLDEN) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def _unit24(x):
+    return np.float32((x >> 40) * 2.0 ** -24)
+
+
+def disc_emission(seed, first_id, count, centre, radius, speed, spread, floor_y):
+    """Particles first_id .. first_id+count-1 of a floor-disc particle source
+    (the benchmark fountain; the reference's own fountain is the fixed 7-particle
+    pattern of src/CBaseParticleSimulator.cpp:187-210). Particle g draws from the
+    SplitMix64 stream at seed + 32*g*golden: up to 15 points (2u-1, 2u-1) of the
+    unit square until one lies in the unit disc (else the centre), then
+    v_y = speed * (1 + spread*(2u-1)) from draw 31. Float32 arithmetic, one IEEE
+    operation at a time. Returns pos (count, 3) and vel (count, 3)."""
+    f = np.float32
+    one, two = f(1.0), f(2.0)
+    cx, cz, r, sp, spr = f(centre[0]), f(centre[1]), f(radius), f(speed), f(spread)
+    pos = np.zeros((count, 3), np.float32)
+    vel = np.zeros((count, 3), np.float32)
+    for q in range(count):
+        g = first_id + q
+        st = (int(seed) + 32 * g * _GOLDEN) & _M64
+        a = b = f(0.0)
+        for t in range(15):
+            u = f(two * _unit24(_splitmix64_at(st, 2 * t + 1)) - one)
+            w = f(two * _unit24(_splitmix64_at(st, 2 * t + 2)) - one)
+            if f(f(u * u) + f(w * w)) <= one:
+                a, b = u, w
+                break
+        pos[q] = (f(cx + f(r * a)), f(floor_y), f(cz + f(r * b)))
+        vel[q, 1] = f(sp * f(one + f(spr * f(two * _unit24(_splitmix64_at(st, 31)) - one))))
+    return pos, vel
+
+
 class OracleSim:
     """The reference CPU simulator, restated (CCPUParticleSimulator)."""
 

@@ -1,0 +1,292 @@
+"""Batched scenes, HIP graphs, kernel variants, the autotuner and device
+emitters of libsph_hip.so, through its C ABI.
+
+Bars:
+  * every scene of a batch is bitwise what it is alone in a context of its own
+    (same positions, velocities, cells, permutation and cell table), and alone
+    contexts are bitwise the canonical-order oracle (test_gpu_parity.py);
+  * graph replays, every kernel variant, neighbour lists on or off, and a state
+    that went through the autotuner give bitwise the same trajectories;
+  * the device fountain is bitwise the oracle's fountain (the reference's
+    generateParticles, src/CBaseParticleSimulator.cpp:187-210), and the device
+    disc emitter is bitwise a host append of the oracle's restatement of it.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = np.float32(0.0457)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_bits_equal(a, b, what):
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    assert a.shape == b.shape, "%s: shapes %s vs %s" % (what, a.shape, b.shape)
+    diff = np.nonzero(bits(a).ravel() != bits(b).ravel())[0]
+    if len(diff):
+        k = diff[0]
+        raise AssertionError("%s: %d of %d values differ, first at flat %d: %r vs %r" %
+                             (what, len(diff), a.size, k, a.ravel()[k], b.ravel()[k]))
+
+
+def small_scenes(hip, count=6, seed=11, nmin=300, nmax=3000):
+    rng = hip.scenes.SplitMix64(seed)
+    return [hip.scenes.batch_scene(rng, nmin, nmax, scene_seed=1000 + i) for i in range(count)]
+
+
+def run_alone(hip, sc, steps):
+    sim = hip.HIPGridParticleSimulator(sc["box"], capacity=sc["n"])
+    sim.upload(sc["pos"])
+    if steps:
+        sim.advance(steps)
+    out = sim.download()
+    sim.close()
+    return out
+
+
+def make_batch(hip, scs, slack=37, **kw):
+    b = hip.SceneBatch([(sc["box"], sc["n"] + slack * i) for i, sc in enumerate(scs)], **kw)
+    for i, sc in enumerate(scs):
+        b.upload(i, sc["pos"])
+    return b
+
+
+def test_batch_equals_alone(hip):
+    scs = small_scenes(hip)
+    print("scene sizes", [sc["n"] for sc in scs], flush=True)
+    with make_batch(hip, scs) as b:
+        b.step(25)
+        for i, sc in enumerate(scs):
+            p, v, ids = b.download(i)
+            pa, va, _ = run_alone(hip, sc, 25)
+            assert_bits_equal(p, pa, "scene %d positions" % i)
+            assert_bits_equal(v, va, "scene %d velocities" % i)
+            assert np.array_equal(ids, np.arange(sc["n"]))
+
+
+def test_batch_grid_tables_equal_alone(hip):
+    """Cells, permutation and cell table of every scene, scene-relative."""
+    lib = hip._lib
+    scs = small_scenes(hip, count=4, seed=5)
+    with make_batch(hip, scs) as b:
+        b.phase(lib.PHASE_GRID)
+        for i, sc in enumerate(scs):
+            sim = hip.HIPGridParticleSimulator(sc["box"], capacity=sc["n"])
+            sim.upload(sc["pos"])
+            sim.updateGrid()
+            for kind in (lib.DEBUG_CELL_PRESORT, lib.DEBUG_PERM, lib.DEBUG_CELL, lib.DEBUG_CELL_START,
+                         lib.DEBUG_IDS):
+                assert np.array_equal(b.debug(i, kind), sim.debug(kind)), "scene %d debug kind %d" % (i, kind)
+            sim.close()
+
+
+def test_batch_vs_oracle(hip, orc):
+    scs = small_scenes(hip, count=3, seed=21, nmin=500, nmax=1500)
+    with make_batch(hip, scs) as b:
+        b.step(10)
+        for i, sc in enumerate(scs):
+            o = orc.OracleSim(sc["box"], sc["n"])
+            o.add_particles(sc["pos"])
+            o.set_order_mode(orc.ORDER_CANONICAL)
+            o.step(10)
+            p, v, _ = b.download(i)
+            assert_bits_equal(p, o.state()["pos"], "scene %d positions vs oracle" % i)
+            assert_bits_equal(v, o.state()["vel"], "scene %d velocities vs oracle" % i)
+            o.close()
+
+
+def test_graph_replays_equal_eager(hip):
+    """Graph replays with eager work (downloads, syncs) between them, every
+    replay checked against the eager trajectory."""
+    scs = small_scenes(hip, count=5, seed=3)
+    with make_batch(hip, scs) as eager, make_batch(hip, scs) as graph:
+        graph.set_graphs(True)
+        for chunk in (1, 1, 2, 5, 1, 8):
+            eager.step(chunk)
+            graph.step(chunk)
+            for i in range(len(scs)):
+                pe, ve, _ = eager.download(i)
+                pg, vg, _ = graph.download(i)
+                assert_bits_equal(pg, pe, "graph scene %d positions" % i)
+                assert_bits_equal(vg, ve, "graph scene %d velocities" % i)
+
+
+def test_single_scene_graphs_equal_eager(hip):
+    pos = hip.scenes.lattice(16, 16, 16, 0.4, jitter_h=0.01, seed=0)
+    a = hip.HIPGridParticleSimulator(0.4, capacity=len(pos))
+    b = hip.HIPGridParticleSimulator(0.4, capacity=len(pos))
+    a.upload(pos)
+    b.upload(pos)
+    hip._lib.check(b._lib.sph_set_graphs(b._ctx, 1))
+    for chunk in (3, 1, 6):
+        a.advance(chunk)
+        b.advance(chunk)
+        assert_bits_equal(b.download()[0], a.download()[0], "graph positions")
+    a.close()
+    b.close()
+
+
+def test_variants_and_lists_identical(hip):
+    scs = small_scenes(hip, count=3, seed=8)
+    ref = None
+    for lists in (False, True):
+        for v in range(hip._lib.NUM_VARIANTS):
+            with make_batch(hip, scs, max_neighbours=64) as b:
+                b.set_variant(v, lists)
+                assert b.variant() == (v, lists)
+                b.step(8)
+                got = [b.download(i)[:2] for i in range(len(scs))]
+            if ref is None:
+                ref = got
+                continue
+            for i in range(len(scs)):
+                assert_bits_equal(got[i][0], ref[i][0], "variant %d lists %d scene %d pos" % (v, lists, i))
+                assert_bits_equal(got[i][1], ref[i][1], "variant %d lists %d scene %d vel" % (v, lists, i))
+
+
+def test_autotune_changes_only_the_choice(hip):
+    scs = small_scenes(hip, count=4, seed=9)
+    with make_batch(hip, scs, max_neighbours=64) as a, make_batch(hip, scs, max_neighbours=64) as b:
+        a.step(3)
+        b.step(3)
+        times = a.autotune(steps=2)
+        print("autotune ms/step:", {k: round(t, 4) for k, t in times.items()}, flush=True)
+        assert len(times) == 2 * hip._lib.NUM_VARIANTS
+        v, lists = a.variant()
+        assert times[(v, lists)] == min(times.values())
+        a.step(10)
+        b.step(10)
+        for i in range(len(scs)):
+            assert_bits_equal(a.download(i)[0], b.download(i)[0], "scene %d after autotune" % i)
+
+
+def test_add_and_remove_scenes(hip):
+    scs = small_scenes(hip, count=4, seed=13, nmin=400, nmax=1200)
+    pool = 256 * 20
+    with hip.SceneBatch(pool=pool, max_scenes=3) as b:
+        ids = [b.add_scene(sc["box"], sc["n"]) for sc in scs[:3]]
+        assert ids == [0, 1, 2]
+        for i in range(3):
+            b.upload(ids[i], scs[i]["pos"])
+        with pytest.raises(hip.SphError):
+            b.add_scene(scs[3]["box"], scs[3]["n"])          # no free id
+        b.step(5)
+        b.remove_scene(1)
+        d = b.add_scene(scs[3]["box"], scs[3]["n"])         # reuses id 1 and the first fitting range
+        assert d == 1
+        b.upload(d, scs[3]["pos"])
+        info = b.info()
+        assert info["scenes"] == 3 and info["pool"] == pool
+        b.step(7)
+        for sid, sc, steps in ((0, scs[0], 12), (2, scs[2], 12), (1, scs[3], 7)):
+            p, v, _ = b.download(sid)
+            pa, va, _ = run_alone(hip, sc, steps)
+            assert_bits_equal(p, pa, "scene %d positions" % sid)
+            assert_bits_equal(v, va, "scene %d velocities" % sid)
+        with pytest.raises(hip.SphError):
+            b.upload(0, np.zeros((scs[0]["n"] + 1, 3), np.float32))   # beyond its capacity
+
+
+def test_device_fountain_matches_oracle(hip, orc):
+    """SPH_EMIT_FOUNTAIN: the reference's 7 particles per step added on the device."""
+    for fused, steps in ((True, 40), (False, 12)):
+        sim = hip.HIPGridParticleSimulator(0.4, scenario=hip.FOUNTAIN, device_emitter=True, fused=fused)
+        sim.setupScene()
+        o = orc.OracleSim(0.4, scenario=orc.FOUNTAIN)
+        o.setup_scene()
+        o.set_order_mode(orc.ORDER_CANONICAL)
+        for _ in range(steps):
+            sim.step()
+        o.step(steps)
+        assert sim.getParticlesCount() == o.count == 7 * steps
+        p, v, ids = sim.download()
+        assert_bits_equal(p, o.state()["pos"], "device fountain positions (fused=%s)" % fused)
+        assert_bits_equal(v, o.state()["vel"], "device fountain velocities (fused=%s)" % fused)
+        sim.close()
+        o.close()
+
+
+def test_device_fountain_stops_at_limit(hip):
+    sim = hip.HIPGridParticleSimulator(0.4, scenario=hip.FOUNTAIN, device_emitter=True, capacity=2000)
+    sim.setupScene()
+    e = hip._lib.sph_emitter()
+    e.kind = hip._lib.EMIT_FOUNTAIN
+    e.limit = 30          # emits while count < 23: 0, 7, 14, 21 -> 28
+    import ctypes as C
+    hip._lib.check(sim._lib.sph_scene_set_emitter(sim._ctx, 0, C.byref(e)))
+    sim.advance(10)
+    assert sim.getParticlesCount() == 28
+
+
+def disc_host_reference(hip, orc, box, em, steps):
+    """The same run with the emitter's particles computed by the oracle's
+    restatement and appended from the host before every step."""
+    sim = hip.HIPGridParticleSimulator(box, capacity=em["limit"])
+    n = 0
+    floor_y = np.float32(-np.float32(box)) / np.float32(2.0)
+    for _ in range(steps):
+        k = min(em["rate"], em["limit"] - n)
+        if k > 0:
+            pos, vel = orc.disc_emission(em["seed"], n, k, em["centre"], em["radius"], em["speed"], em["spread"],
+                                         floor_y)
+            if n == 0:
+                sim.upload(pos, vel)
+            else:
+                sim.append(pos, vel)
+            n += k
+        sim.advance(1)
+    return sim.download(), n
+
+
+def test_disc_emitter_matches_host_append(hip, orc):
+    box = 0.6
+    em = dict(rate=53, limit=450, centre=(0.01, -0.02), radius=float(4 * H), speed=2.0, spread=0.1, seed=77)
+    steps = 14
+    for graphs in (False, True):
+        with hip.SceneBatch([(box, em["limit"])]) as b:
+            b.set_graphs(graphs)
+            b.set_emitter(0, kind=hip._lib.EMIT_DISC, **em)
+            for _ in range(steps):
+                b.step(1)
+            p, v, ids = b.download(0)
+        (pr, vr, _), n = disc_host_reference(hip, orc, box, em, steps)
+        assert len(p) == n == 450
+        assert_bits_equal(p, pr, "disc emitter positions (graphs=%s)" % graphs)
+        assert_bits_equal(v, vr, "disc emitter velocities (graphs=%s)" % graphs)
+
+
+def test_emitter_scene_in_batch_equals_alone(hip):
+    """One emitting scene among others: each scene as alone."""
+    scs = small_scenes(hip, count=3, seed=17)
+    em = dict(rate=40, limit=300, centre=(0.0, 0.0), radius=float(3 * H), speed=1.5, spread=0.1, seed=3)
+    with hip.SceneBatch([(sc["box"], sc["n"]) for sc in scs] + [(0.5, em["limit"])]) as b:
+        for i, sc in enumerate(scs):
+            b.upload(i, sc["pos"])
+        b.set_emitter(3, kind=hip._lib.EMIT_DISC, **em)
+        b.set_graphs(True)
+        b.step(9)
+        for i, sc in enumerate(scs):
+            assert_bits_equal(b.download(i)[0], run_alone(hip, sc, 9)[0], "scene %d" % i)
+        with hip.SceneBatch([(0.5, em["limit"])]) as alone:
+            alone.set_emitter(0, kind=hip._lib.EMIT_DISC, **em)
+            alone.step(9)
+            assert_bits_equal(b.download(3)[0], alone.download(0)[0], "emitter scene")
+            assert b.count(3) == alone.count(0) == 300
+
+
+def test_batch_errors(hip):
+    with hip.SceneBatch([(0.4, 100)]) as b:
+        with pytest.raises(hip.SphError):
+            b.step(1)                       # nothing uploaded
+        with pytest.raises(hip.SphError):
+            b.set_variant(hip._lib.NUM_VARIANTS)
+        with pytest.raises(hip.SphError):
+            b.set_variant(0, lists=True)    # created without lists
+        with pytest.raises(hip.SphError):
+            b.download(1)                   # no scene 1
+        with pytest.raises(hip.SphError):
+            b.set_emitter(0, kind=7)
