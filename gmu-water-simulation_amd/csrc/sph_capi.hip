@@ -112,7 +112,7 @@ struct sph_ctx {
     uint32_t *nlist_cnt = nullptr;
     uint32_t kmax = 0;           // list capacity allocated
     bool use_lists = false;
-    int variant = 0;
+    int variant = sph::kDefaultVariant;
 
     bool keys_valid = false;   // key_h holds the cells of the current positions
     int next_phase = SPH_PHASE_GRID;
@@ -195,6 +195,11 @@ Consts make_consts()
     k.rho0 = kRestDensity;
     k.wall_k_f = (float) kWallK;
     return k;
+}
+
+sph_status report(sph_status code, const std::string &msg)
+{
+    return fail(code, msg);
 }
 
 }  // namespace sph
@@ -1045,6 +1050,15 @@ void sph_default_scene(sph_scene_desc *d, float box, uint32_t capacity)
     d->gravity[1] = sph::kGravityY;
     d->dt = sph::kDt;
     d->capacity = capacity;
+}
+
+uint64_t sph_slot_bytes(uint32_t max_neighbours)
+{
+    const uint64_t lists = max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0 : max_neighbours;
+    // pos/vel ping-pong 48, ids 8, keys and values 20, rho 4, p/rho^2 8, 1/rho 8,
+    // records 64, accelerations 12, counts 8, list entries, and 4 for the
+    // histograms, the group map and the scene's share of the cell table
+    return 48 + 8 + 20 + 4 + 8 + 8 + 64 + 12 + 8 + 4 * lists + 4;
 }
 
 int sph_grid_res(float box)

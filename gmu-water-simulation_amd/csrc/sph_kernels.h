@@ -55,7 +55,8 @@ struct NeighbourVariant {
     int threads;
     int unroll;
 };
-constexpr int kNumVariants = 6;
+constexpr int kNumVariants = 8;
+constexpr int kDefaultVariant = 2;   // 64 threads, 4 candidates: fastest measured on both workloads
 extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 
 // Emitters (one workgroup each) add particles at the start of a step and update

@@ -16,6 +16,7 @@
 // a segment of cell ids, and one launch of every stage covers all of them.
 
 #include "sph_kernels.h"
+#include "sph_math.h"
 
 #include <climits>
 
@@ -43,18 +44,6 @@ __device__ __forceinline__ uint32_t hash_key(float px, float py, float pz, const
     const int y = clamp_cell(to_int_x86(floor(((double) py + (double) s.by / 2.0) / h)), s.ry);
     const int z = clamp_cell(to_int_x86(floor(((double) pz + (double) s.bz / 2.0) / h)), s.rz);
     return s.coff + (uint32_t) (x + y * s.rx + z * s.rx * s.ry);
-}
-
-// t^3 correctly rounded (error-free products via FMA), the value libm's
-// pow(t, 3.0) returns (src/CCPUParticleSimulator.cpp:14).
-__device__ __forceinline__ double cube_cr(double t)
-{
-    const double p = t * t;
-    const double e = __builtin_fma(t, t, -p);   // t*t = p + e exactly
-    const double q = p * t;
-    const double f = __builtin_fma(p, t, -q);   // p*t = q + f exactly
-    const double c = __builtin_fma(e, t, f);    // f + e*t
-    return q + c;
 }
 
 // CCollisionGeometry::inverseBoundingBoxBounce (src/CCollisionGeometry.cpp:117-132)
@@ -521,44 +510,6 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
     cnt = 0;
 }
 
-// Float quotients a_c / b correctly rounded, from double products a_c * rb with
-// rb ~ 1/b to within 2^-52: for float a, b with a normal-range quotient the exact
-// a/b is never within 2^-49 (relative) of a float rounding midpoint (a = b*m for
-// a 25-bit midpoint m is impossible, and otherwise |a - b*m| is at least one unit
-// of b*m's last bit), so the rounded product is the IEEE float quotient. The
-// quotients of the force pass cannot overflow for finite operands (|grad| <=
-// |spiky| h^2, |visc term| <= |dv| visc h / 328 with density >= its self term),
-// and infinities and NaNs propagate alike; only quotients that would be
-// subnormal take the IEEE division itself.
-__device__ __forceinline__ bool subnormal_quot(double q)
-{
-    return fabs(q) < 0x1p-126 && q != 0.0;
-}
-
-// 1/b to within one rounding: hardware estimate plus two Newton-Raphson steps.
-__device__ __forceinline__ double rcp_refined(double b)
-{
-    double r = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-b, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-
-__device__ __forceinline__ void div3(float a0, float a1, float a2, float b, double rb, float &q0, float &q1,
-                                     float &q2)
-{
-    const double d0 = (double) a0 * rb, d1 = (double) a1 * rb, d2 = (double) a2 * rb;
-    q0 = (float) d0;
-    q1 = (float) d1;
-    q2 = (float) d2;
-    if (__builtin_expect((int) subnormal_quot(d0) | (int) subnormal_quot(d1) | (int) subnormal_quot(d2), 0)) {
-        q0 = a0 / b;
-        q1 = a1 / b;
-        q2 = a2 / b;
-    }
-}
-
 // updateDensityPressure, src/CCPUParticleSimulator.cpp:93-141. Also stores
 // p/rho^2 (double, as src/CCPUParticleSimulator.cpp:181 forms it), 1/rho
 // (double) and, with kLists, the particle's accepted neighbours (minus itself)
@@ -657,29 +608,19 @@ __device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, c
 {
     const float dx = xi - n.x, dyv = yi - n.y, dzv = zi - n.z;
     const float r2 = dx * dx + dyv * dyv + dzv * dzv;
-    const double r = sqrt((double) r2);
+    const double r = sqrt_of_float((double) r2);
     const double hr = k.h - r;
     const float sg = (float) (k.spiky * (hr * hr));
-    const float rf = (float) r;
     float gx, gy, gz;
-    div3(dx * sg, dyv * sg, dzv * sg, rf, rcp_refined((double) rf), gx, gy, gz);
+    div3_small_divisor(dx * sg, dyv * sg, dzv * sg, (float) r, gx, gy, gz);
     const float pf = (float) (ai + n.prr);
     a.px = a.px + gx * pf;
     a.py = a.py + gy * pf;
     a.pz = a.pz + gz * pf;
     const float lap = (float) (k.visc * hr);
     float tx, ty, tz;
-    const float t0 = (n.vx - vxi) * lap, t1 = (n.vy - vyi) * lap, t2 = (n.vz - vzi) * lap;
-    const double d0 = (double) t0 * n.rinv, d1 = (double) t1 * n.rinv, d2 = (double) t2 * n.rinv;
-    tx = (float) d0;
-    ty = (float) d1;
-    tz = (float) d2;
-    if (__builtin_expect((int) subnormal_quot(d0) | (int) subnormal_quot(d1) | (int) subnormal_quot(d2), 0)) {
-        const float rj = ldf(bv.rho, n.j * 4u);
-        tx = t0 / rj;
-        ty = t1 / rj;
-        tz = t2 / rj;
-    }
+    div3_rcp((n.vx - vxi) * lap, (n.vy - vyi) * lap, (n.vz - vzi) * lap, n.rinv,
+             [&]() { return ldf(bv.rho, n.j * 4u); }, tx, ty, tz);
     a.vx = a.vx + tx;
     a.vy = a.vy + ty;
     a.vz = a.vz + tz;
@@ -726,13 +667,16 @@ __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *
                 cnt += valid && r2 <= k.hsq_f && j != li;
             },
             [&]() {
-                if (cnt == 0) return;
-                PairData nxt = load_pair(bv, my[0]);
-                for (int q = 0; q < cnt; ++q) {
-                    const PairData cur = nxt;
-                    if (q + 1 < cnt) nxt = load_pair(bv, my[(q + 1) * T]);
-                    force_pair(bv, k, cur, xi, yi, zi, vxi, vyi, vzi, ai, a);
+                // two pairs in flight: the loads of one overlap the arithmetic of the other
+                int q = 0;
+                PairData pa = load_pair(bv, my[0]), pb;
+                for (; q + 1 < cnt; q += 2) {
+                    pb = load_pair(bv, my[(q + 1) * T]);
+                    force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
+                    if (q + 2 < cnt) pa = load_pair(bv, my[(q + 2) * T]);
+                    force_pair(bv, k, pb, xi, yi, zi, vxi, vyi, vzi, ai, a);
                 }
+                if (q < cnt) force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
             });
     }
     const float fac = k.neg_mass * rhoi;
@@ -835,7 +779,8 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     const float vxi = p.vx[i], vyi = p.vy[i], vzi = p.vz[i];
     float a0, a1, a2;
-    sph_accel<T, U>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1, a2);
+    sph_accel<T, U>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, a0,
+                            a1, a2);
     wall_accel(xi, yi, zi, vxi, vyi, vzi, s, k, a0, a1, a2);
     if (kStoreAcc) {
         ax[i] = a0;
@@ -854,19 +799,20 @@ inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 // The compiled shapes of the two neighbour kernels, indexed by variant.
 template <int T, int U>
 struct NeighbourKernels {
-    static void density(const dim3 &g, hipStream_t st, bool lists, const ParticleSoA &p, const uint32_t *cell,
-                        const uint32_t *start, const DensityFields &f, const NeighbourList &nl, const SceneTable &t,
-                        const Consts &k)
+    static void density(hipStream_t st, bool lists, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
+                        const DensityFields &f, const NeighbourList &nl, const SceneTable &t, const Consts &k)
     {
+        const dim3 g(blocks_for(t.nslots, T));
         if (lists)
             hipLaunchKernelGGL((k_density<T, U, true>), g, dim3(T), 0, st, p, cell, start, f, nl, t, k);
         else
             hipLaunchKernelGGL((k_density<T, U, false>), g, dim3(T), 0, st, p, cell, start, f, nl, t, k);
     }
-    static void fused(const dim3 &g, hipStream_t st, const ParticleSoA &p, const uint32_t *cell,
-                      const uint32_t *start, const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out,
-                      uint32_t *key_next, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k)
+    static void fused(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
+                      const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out, uint32_t *key_next,
+                      float *ax, float *ay, float *az, const SceneTable &t, const Consts &k)
     {
+        const dim3 g(blocks_for(t.nslots, T));
         if (ax)
             hipLaunchKernelGGL((k_forces_fused<T, U, true>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
                                key_next, ax, ay, az, t, k);
@@ -876,11 +822,22 @@ struct NeighbourKernels {
     }
 };
 
+// X(index, threads, unroll)
+#define SPH_VARIANTS(X) \
+    X(0, 256, 4)        \
+    X(1, 128, 4)        \
+    X(2, 64, 4)         \
+    X(3, 256, 2)        \
+    X(4, 128, 2)        \
+    X(5, 256, 8)        \
+    X(6, 64, 2)         \
+    X(7, 64, 8)
+
 }  // namespace
 
-const NeighbourVariant kNeighbourVariants[kNumVariants] = {
-    {256, 4}, {128, 4}, {64, 4}, {256, 2}, {128, 2}, {256, 8},
-};
+#define SPH_VARIANT_ROW(i, t, u) {t, u},
+const NeighbourVariant kNeighbourVariants[kNumVariants] = {SPH_VARIANTS(SPH_VARIANT_ROW)};
+#undef SPH_VARIANT_ROW
 
 // ---------------------------------------------------------------- launchers
 
@@ -935,12 +892,11 @@ void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *
     if (t.nslots == 0) return;
     const bool lists = nl.kmax != 0;
     switch (variant) {
-    case 1: NeighbourKernels<128, 4>::density(dim3(blocks_for(t.nslots, 128)), st, lists, p, cell, start, f, nl, t, k); break;
-    case 2: NeighbourKernels<64, 4>::density(dim3(blocks_for(t.nslots, 64)), st, lists, p, cell, start, f, nl, t, k); break;
-    case 3: NeighbourKernels<256, 2>::density(dim3(blocks_for(t.nslots, 256)), st, lists, p, cell, start, f, nl, t, k); break;
-    case 4: NeighbourKernels<128, 2>::density(dim3(blocks_for(t.nslots, 128)), st, lists, p, cell, start, f, nl, t, k); break;
-    case 5: NeighbourKernels<256, 8>::density(dim3(blocks_for(t.nslots, 256)), st, lists, p, cell, start, f, nl, t, k); break;
-    default: NeighbourKernels<256, 4>::density(dim3(blocks_for(t.nslots, 256)), st, lists, p, cell, start, f, nl, t, k); break;
+#define SPH_DENSITY_CASE(i, t_, u) \
+    case i: NeighbourKernels<t_, u>::density(st, lists, p, cell, start, f, nl, t, k); break;
+        SPH_VARIANTS(SPH_DENSITY_CASE)
+#undef SPH_DENSITY_CASE
+    default: NeighbourKernels<64, 4>::density(st, lists, p, cell, start, f, nl, t, k); break;
     }
 }
 
@@ -974,12 +930,11 @@ void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint3
 {
     if (t.nslots == 0) return;
     switch (variant) {
-    case 1: NeighbourKernels<128, 4>::fused(dim3(blocks_for(t.nslots, 128)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
-    case 2: NeighbourKernels<64, 4>::fused(dim3(blocks_for(t.nslots, 64)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
-    case 3: NeighbourKernels<256, 2>::fused(dim3(blocks_for(t.nslots, 256)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
-    case 4: NeighbourKernels<128, 2>::fused(dim3(blocks_for(t.nslots, 128)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
-    case 5: NeighbourKernels<256, 8>::fused(dim3(blocks_for(t.nslots, 256)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
-    default: NeighbourKernels<256, 4>::fused(dim3(blocks_for(t.nslots, 256)), st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+#define SPH_FUSED_CASE(i, t_, u) \
+    case i: NeighbourKernels<t_, u>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+        SPH_VARIANTS(SPH_FUSED_CASE)
+#undef SPH_FUSED_CASE
+    default: NeighbourKernels<64, 4>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
     }
 }
 

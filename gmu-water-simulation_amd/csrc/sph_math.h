@@ -1,0 +1,107 @@
+// sph_math.h -- device arithmetic of the grid-SPH kernels whose results must be
+// the IEEE / libm results the reference's CPU code gets (src/CCPUParticleSimulator.cpp),
+// bit for bit. Shared by the kernels and by the exhaustive device checks of them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+// t^3 correctly rounded (error-free products via FMA), the value libm's
+// pow(t, 3.0) returns (src/CCPUParticleSimulator.cpp:14).
+__device__ __forceinline__ double cube_cr(double t)
+{
+    const double p = t * t;
+    const double e = __builtin_fma(t, t, -p);   // t*t = p + e exactly
+    const double q = p * t;
+    const double f = __builtin_fma(p, t, -q);   // p*t = q + f exactly
+    const double c = __builtin_fma(e, t, f);    // f + e*t
+    return q + c;
+}
+
+// sqrt(x), correctly rounded, for x the double of a finite non-negative float
+// (the r^2 of an accepted pair, src/CCPUParticleSimulator.cpp:20). The
+// Goldschmidt refinement of v_rsq_f64 that the compiler emits for a double
+// sqrt, without its rescaling of arguments below 2^-767 (a positive float is at
+// least 2^-149) and with sqrt(0) = 0 selected. Checked against sqrt() for every
+// float in [0, h^2] on the device.
+__device__ __forceinline__ double sqrt_of_float(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, hh = y * 0.5;
+    const double r = __builtin_fma(-hh, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    hh = __builtin_fma(hh, r, hh);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, hh, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, hh, g);
+    return x == 0.0 ? 0.0 : g;
+}
+
+// 1/b to within one rounding: hardware estimate plus two Newton-Raphson steps.
+__device__ __forceinline__ double rcp_refined(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// Float quotients a/b from double products a * rb, rb ~ 1/b to within 2^-52.
+// For float a, b whose quotient rounds to a normal float, the exact a/b is never
+// within 2^-49 (relative) of a float rounding midpoint: a = b*m for a 25-bit
+// midpoint m is impossible, a 24-bit midpoint (the one between the largest
+// subnormal and the smallest normal) is hit exactly only when b is a power of
+// two, where rb and the product are exact, and otherwise |a - b*m| is at least
+// one unit of b*m's last bit. So the rounded product is the IEEE quotient. A
+// result that rounds to zero or to a subnormal is redone with the IEEE division.
+__device__ __forceinline__ bool zero_or_subnormal(float q)
+{
+    return __builtin_amdgcn_classf(q, 0x0F0);   // -subnormal, -0, +0, +subnormal
+}
+
+__device__ __forceinline__ bool subnormal(float q)
+{
+    return __builtin_amdgcn_classf(q, 0x090);   // -subnormal, +subnormal
+}
+
+// a_c / b for b in (0, 1): |a/b| > |a| >= 2^-149 for a != 0, so a zero result
+// means a = 0 (exact); only subnormal results take the IEEE division.
+__device__ __forceinline__ void div3_small_divisor(float a0, float a1, float a2, float b, float &q0, float &q1,
+                                                  float &q2)
+{
+    const double rb = rcp_refined((double) b);
+    q0 = (float) ((double) a0 * rb);
+    q1 = (float) ((double) a1 * rb);
+    q2 = (float) ((double) a2 * rb);
+    if (__builtin_expect((int) subnormal(q0) | (int) subnormal(q1) | (int) subnormal(q2), 0)) {
+        q0 = a0 / b;
+        q1 = a1 / b;
+        q2 = a2 / b;
+    }
+}
+
+// a_c / b with rb = 1/b rounded to double: zero or subnormal results of a
+// non-zero a take the IEEE division (b is re-read through `divisor` only then).
+template <typename Divisor>
+__device__ __forceinline__ void div3_rcp(float a0, float a1, float a2, double rb, Divisor divisor, float &q0,
+                                         float &q1, float &q2)
+{
+    q0 = (float) ((double) a0 * rb);
+    q1 = (float) ((double) a1 * rb);
+    q2 = (float) ((double) a2 * rb);
+    const bool t0 = zero_or_subnormal(q0) & (a0 != 0.0f);
+    const bool t1 = zero_or_subnormal(q1) & (a1 != 0.0f);
+    const bool t2 = zero_or_subnormal(q2) & (a2 != 0.0f);
+    if (__builtin_expect((int) t0 | (int) t1 | (int) t2, 0)) {
+        const float b = divisor();
+        q0 = a0 / b;
+        q1 = a1 / b;
+        q2 = a2 / b;
+    }
+}
+
+}  // namespace sph

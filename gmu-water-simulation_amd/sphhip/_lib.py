@@ -21,7 +21,8 @@ NEIGHBOUR_LISTS_OFF = 0xFFFFFFFF
  DEBUG_PRESSURE, DEBUG_ACCEL, DEBUG_IDS) = range(9)
 STAGES = ["hash", "sort", "cells", "reorder", "density", "forces", "collisions", "integrate"]
 EMIT_FOUNTAIN, EMIT_DISC = 0, 1
-NUM_VARIANTS = 6
+NUM_VARIANTS = 8
+DEFAULT_VARIANT = 2
 GROUP_SLOTS = 256
 
 # Every entry point of include/sph_hip.h with its ctypes signature.
@@ -67,6 +68,14 @@ SIGNATURES = {
     "sph_get_variant": (C.c_int, [_vp, _vp, _vp]),
     "sph_variant_shape": (C.c_int, [C.c_int, _vp, _vp]),
     "sph_autotune": (C.c_int, [_vp, _u32, _vp]),
+    "sph_slot_bytes": (C.c_uint64, [_u32]),
+    "sph_server_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sph_server_destroy": (None, [_vp]),
+    "sph_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
+    "sph_server_job": (C.c_int, [_vp, C.c_uint64, _vp]),
+    "sph_server_wait": (C.c_int, [_vp, C.c_uint64, C.c_double, _vp]),
+    "sph_server_result": (C.c_int, [_vp, C.c_uint64, _vp, _vp, _u32]),
+    "sph_server_stats_get": (C.c_int, [_vp, _vp]),
 }
 
 

@@ -191,6 +191,7 @@ class SceneBatch:
 
 
 def variant_shape(variant):
+    """(threads per workgroup, candidates per lane per trip)."""
     t, u = C.c_int(0), C.c_int(0)
     _lib.check(_lib.load().sph_variant_shape(int(variant), C.byref(t), C.byref(u)))
     return t.value, u.value

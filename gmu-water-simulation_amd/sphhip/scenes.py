@@ -67,14 +67,15 @@ class SplitMix64:
 HALF = float(np.float32(H / np.float32(2.0)))
 
 
-def batch_scene(rng, nmin=2048, nmax=65536, scene_seed=0):
+def batch_scene(rng, nmin=2048, nmax=65536, scene_seed=0, kind=None):
     """One scene of the batched workload: N log-uniform in [nmin, nmax], a dam
-    break or a dropped cuboid, per-axis box aspect U[0.5, 2] with the volume
-    scaled to N so that the particles fill about 1/4 of the box (the fill of the
-    reference dam break). Returns dict(box, pos, kind, n)."""
+    break or a dropped cuboid (drawn, or `kind`), per-axis box aspect U[0.5, 2]
+    with the volume scaled to N so that the particles fill about 1/4 of the box
+    (the fill of the reference dam break). Returns dict(box, pos, kind, n)."""
     u = [rng.uniform() for _ in range(6)]
     n = int(round(np.exp(np.log(nmin) + u[0] * (np.log(nmax) - np.log(nmin)))))
-    kind = "dam_break" if u[1] < 0.5 else "cuboid_drop"
+    if kind is None:
+        kind = "dam_break" if u[1] < 0.5 else "cuboid_drop"
     aspect = np.array([0.5 + 1.5 * u[2], 0.5 + 1.5 * u[3], 0.5 + 1.5 * u[4]])
     volume = 4.0 * n * HALF ** 3
     box = (aspect * (volume / aspect.prod()) ** (1.0 / 3.0)).astype(np.float32)

@@ -277,12 +277,13 @@ sph_status sph_scene_set_emitter(sph_ctx *ctx, uint32_t scene, const sph_emitter
 sph_status sph_set_graphs(sph_ctx *ctx, int enable);
 
 /* Shape of the neighbour kernels: variant 0..SPH_NUM_VARIANTS-1 (workgroup size and
- * candidates per trip, see sph_variant_shape) and lists != 0 to keep neighbour lists
+ * candidates per trip, see sph_variant_shape; a new context uses variant 2) and lists != 0 to keep neighbour lists
  * (only if the context was created with max_neighbours > 0). Results are identical for
  * every choice. */
-#define SPH_NUM_VARIANTS 6
+#define SPH_NUM_VARIANTS 8
 sph_status sph_set_variant(sph_ctx *ctx, int variant, int lists);
 sph_status sph_get_variant(const sph_ctx *ctx, int *variant, int *lists);
+/* Threads per workgroup and candidates tested per lane per trip of a variant. */
 sph_status sph_variant_shape(int variant, int *threads, int *unroll);
 
 /* Times `steps` steps of every (variant, lists) choice from the current state, keeps the
@@ -290,6 +291,71 @@ sph_status sph_variant_shape(int variant, int *threads, int *unroll);
  * (may be NULL) receives 2*SPH_NUM_VARIANTS ms-per-step figures, index 2*variant + lists,
  * negative where a choice was not run. */
 sph_status sph_autotune(sph_ctx *ctx, uint32_t steps, float *ms_out);
+
+/* Device bytes one particle slot of a context costs (particle state, sort buffers,
+ * density fields, records, lists of max_neighbours entries, its share of the cell
+ * table), for sizing a pool against an HBM budget. */
+uint64_t sph_slot_bytes(uint32_t max_neighbours);
+
+/* ------------------------------------------------------------------------------
+ * Serving. No counterpart in the reference, whose QTimer/doWork loop
+ * (src/CBaseParticleSimulator.cpp:35, 146-152) drives one scene: a server owns one
+ * batch context and a scheduler thread that admits "advance this scene by k steps"
+ * jobs as they arrive and slots free up (continuous batching), advances every running
+ * job with the batched step, and retires each job after exactly its k steps, so its
+ * result is bitwise the result it has alone. Submission, polling and result calls are
+ * thread-safe; all device work runs on the scheduler thread.
+ * ------------------------------------------------------------------------------ */
+
+typedef struct sph_server sph_server;
+
+typedef struct {
+    uint64_t hbm_budget;       /* device bytes for the pool: slots = budget / sph_slot_bytes */
+    uint32_t max_jobs;         /* jobs resident at once (scene ids) */
+    uint32_t chunk_steps;      /* most steps between two admission points (0 = 8) */
+    uint32_t max_neighbours;   /* as in sph_params */
+    uint32_t flags;            /* SPH_FLAG_* of the context */
+    int32_t variant;           /* neighbour-kernel variant, -1 = default */
+    uint32_t graphs;           /* != 0: replay steps from HIP graphs */
+} sph_server_params;
+
+typedef struct {
+    sph_scene_desc scene;      /* box, gravity, dt, capacity (>= n and the emitter's limit) */
+    const float *pos3;         /* n initial positions (copied at submission) */
+    const float *vel3;         /* NULL = at rest */
+    uint32_t n;
+    uint32_t steps;            /* steps to advance */
+    const sph_emitter *emitter; /* NULL = none */
+} sph_job_desc;
+
+enum { SPH_JOB_QUEUED = 0, SPH_JOB_RUNNING = 1, SPH_JOB_DONE = 2, SPH_JOB_FAILED = 3 };
+
+typedef struct {
+    int state;                 /* SPH_JOB_* */
+    uint32_t count;            /* particles (final count once done) */
+    double submit_s, admit_s, done_s;   /* server clock, seconds */
+} sph_job_info;
+
+typedef struct {
+    uint64_t submitted, completed, failed, queued, running;
+    uint64_t particle_steps;   /* sum over batched steps of the live particles advanced */
+    uint64_t batch_steps;      /* batched steps run */
+    double busy_s;             /* wall time inside step chunks */
+    double elapsed_s;          /* server clock now */
+    uint32_t pool, free_slots;
+} sph_server_stats;
+
+sph_status sph_server_create(const sph_server_params *params, sph_server **out);
+/* Stops the scheduler (jobs not done are dropped) and frees everything. */
+void sph_server_destroy(sph_server *srv);
+/* Queues a job; fails with SPH_ERR_CAPACITY if it could never fit the pool. */
+sph_status sph_server_submit(sph_server *srv, const sph_job_desc *job, uint64_t *job_id);
+sph_status sph_server_job(sph_server *srv, uint64_t job_id, sph_job_info *info);
+/* Waits until the job is done or failed (timeout_s < 0: no limit); *info as sph_server_job. */
+sph_status sph_server_wait(sph_server *srv, uint64_t job_id, double timeout_s, sph_job_info *info);
+/* Copies a done job's particles by id (n = its final count) and forgets the job. */
+sph_status sph_server_result(sph_server *srv, uint64_t job_id, float *pos3, float *vel3, uint32_t n);
+sph_status sph_server_stats_get(sph_server *srv, sph_server_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,46 @@
+"""Device checks of the exactness-critical arithmetic of the kernels
+(gmu-water-simulation_amd/csrc/sph_math.h), through tests/hip/libsph_mathcheck.so:
+
+  * sqrt_of_float, the force pass's double sqrt of r^2 (src/CCPUParticleSimulator.cpp:20),
+    equals the compiler's correctly rounded sqrt for EVERY float r^2 in [0, h^2]
+    (all 990,437,133 bit patterns): the whole input domain of an accepted pair;
+  * the double-reciprocal float quotients (pressure gradient / r and viscosity
+    term / rho_j, src/CCPUParticleSimulator.cpp:21, 28, 181-182) equal the IEEE
+    float division on 2^28 random cases per run, with numerators down to 2^-149 so
+    that many quotients are zero, subnormal or the smallest normals.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "hip", "libsph_mathcheck.so")
+HSQ_BITS = 0x3B08DF0C   # fl(h*h), the acceptance threshold
+
+
+@pytest.fixture(scope="module")
+def mc():
+    if not os.path.exists(LIB):
+        pytest.fail("tests/hip/libsph_mathcheck.so is not built (make -C tests/hip)")
+    lib = C.CDLL(LIB)
+    lib.mc_sqrt.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
+    lib.mc_div.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
+    return lib
+
+
+def test_sqrt_exhaustive_over_accepted_r2(mc):
+    assert np.float32(0.0457) * np.float32(0.0457) == np.uint32(HSQ_BITS).view(np.float32)
+    bad, first = C.c_ulonglong(0), C.c_ulonglong(0)
+    assert mc.mc_sqrt(0, HSQ_BITS, C.byref(bad), C.byref(first)) == 0
+    assert bad.value == 0, "%d mismatches, first at float bits 0x%08x" % (bad.value, first.value)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_quotients_match_ieee_division(mc, seed):
+    bad, first = C.c_ulonglong(0), C.c_ulonglong(0)
+    assert mc.mc_div(seed, 1 << 28, C.byref(bad), C.byref(first)) == 0
+    assert bad.value == 0, "%d mismatches, first at case %d" % (bad.value, first.value)
