@@ -1,12 +1,23 @@
 """Benchmark of the MI355X grid-SPH step (libsph_hip.so).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dam_break_1m]
+                    [--batched-steps 500] [--serve-seconds 30]
 
 Runs W untimed steps, then times exactly K fused steps (grid + density + forces
 + walls + integrate) of the workload on one GPU with the state resident in HBM,
 and prints one JSON line: particle-steps/s, ms per step, the roofline of the
 dominant kernel (timed live with HIP events on the library's stream) and the
 CPU oracle timed on this host on a bounded sample of the same workload.
+
+The same line carries the batched and served forms of the step:
+  * "batched": 512 independent scenes (2,048-65,536 particles each, log-uniform,
+    seed 3) advanced together for 500 steps (autotuned kernel shape, HIP graph
+    replay), against the same scenes stepped by plain per-scene launches (one
+    context and stream per scene, a bounded sample of steps);
+  * "serving": Poisson arrivals of dam-break, fountain and cuboid-drop jobs
+    (4,096-1,048,576 particles, 100-2,000 steps) into the native continuous-
+    batching server for a bounded time; p50/p99 job latency and aggregate
+    particle-steps/s.
 """
 import argparse
 import json
@@ -94,6 +105,73 @@ def load_traffic(workload):
         return None
 
 
+def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
+    """The batched workload: one context for all scenes vs one context per scene."""
+    from sphhip import _lib
+    lib = _lib.load()
+    w = sphhip.scenes.batch_workload(nscenes=nscenes, seed=seed)
+    total = sum(s["n"] for s in w)
+    b = sphhip.SceneBatch([(s["box"], s["n"]) for s in w])
+    for i, s in enumerate(w):
+        b.upload(i, s["pos"])
+    b.step(2)
+    tune = b.autotune(steps=3)
+    variant, lists = b.variant()
+    b.set_graphs(True)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    b.close()
+    out = {"workload": "%d scenes, 2,048-65,536 particles each (log-uniform, seed %d), dam break or cuboid drop, "
+                       "boxes at 1/4 fill with per-axis aspect U[0.5, 2]" % (nscenes, seed),
+           "scenes": nscenes, "particles": total, "steps": steps, "value": total * steps / wall,
+           "unit": "particle-steps/s", "ms_per_step": wall * 1e3 / steps, "graphs": True,
+           "variant": {"index": variant, "threads_unroll": sphhip.variant_shape(variant), "lists": lists},
+           "autotune_ms_per_step": {"%d%s" % (v, "L" if l else ""): round(ms, 4) for (v, l), ms in tune.items()}}
+    if per_scene_steps:
+        sims = []
+        for s in w:
+            sim = sphhip.HIPGridParticleSimulator(s["box"], capacity=s["n"])
+            sim.upload(s["pos"])
+            sims.append(sim)
+        for sim in sims:
+            lib.sph_step(sim._ctx, 2, None)
+        for sim in sims:
+            sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(per_scene_steps):
+            for sim in sims:
+                _lib.check(lib.sph_step(sim._ctx, 1, None))
+        for sim in sims:
+            sim.sync()
+        pw = time.perf_counter() - t0
+        for sim in sims:
+            sim.close()
+        ps = total * per_scene_steps / pw
+        out["per_scene"] = {"value": ps, "ms_per_step": pw * 1e3 / per_scene_steps, "steps": per_scene_steps,
+                            "how": "one context and stream per scene, sph_step(ctx, 1) per scene per step from one "
+                                   "host thread, steps 2-%d" % (per_scene_steps + 1)}
+        out["speedup_vs_per_scene"] = out["value"] / ps
+    return out
+
+
+def bench_serving(sphhip, seconds, rate, budget_gb, seed=4):
+    budget = int(budget_gb * 2 ** 30)
+    with sphhip.Server(budget, max_jobs=1024, chunk_steps=8) as srv:
+        recs, st = sphhip.serving.poisson_run(srv, rate, seconds, seed=seed)
+    lat = np.array([r["latency"] for r in recs]) if recs else np.zeros(1)
+    return {"workload": "Poisson arrivals at %.2f jobs/s for %.0f s (seed %d): dam break, fountain or cuboid drop, "
+                        "4,096-1,048,576 particles (log-uniform), 100-2,000 steps (uniform)" % (rate, seconds, seed),
+            "jobs": len(recs), "completed": st["completed"], "failed": st["failed"],
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "max_latency_s": float(lat.max()),
+            "value": st["particle_steps"] / st["elapsed_s"], "unit": "particle-steps/s",
+            "busy_fraction": st["busy_s"] / st["elapsed_s"], "hbm_budget_bytes": budget, "pool_slots": st["pool"],
+            "elapsed_s": st["elapsed_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +180,11 @@ def main():
     ap.add_argument("--workload", default="dam_break_1m")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=500, help="0 skips the batched measurement")
+    ap.add_argument("--per-scene-steps", type=int, default=20)
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="0 skips the serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scales HBM counters)")
     args = ap.parse_args()
@@ -158,7 +241,7 @@ def main():
     prof.close()
 
     line = {
-        "metric": "particle-steps/s on 1M dam break",
+        "metric": "particle-steps/s on 1M dam break; batched-serve agg throughput + p99 latency",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -177,12 +260,18 @@ def main():
         "stage_ms": stages,
         "roofline": roofline,
     }
+    sim.close()
+    if args.batched_steps:
+        print("bench: batched workload, %d steps" % args.batched_steps, file=sys.stderr, flush=True)
+        line["batched"] = bench_batched(sphhip, args.batched_steps, args.per_scene_steps)
+    if args.serve_seconds:
+        print("bench: serving for %.0f s" % args.serve_seconds, file=sys.stderr, flush=True)
+        line["serving"] = bench_serving(sphhip, args.serve_seconds, args.serve_rate, args.serve_budget_gb)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on %d steps" % args.cpu_steps, file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(pos, box, args.cpu_steps)
     else:
         line["cpu_baseline"] = None
-    sim.close()
     print(json.dumps(line), flush=True)
 
 

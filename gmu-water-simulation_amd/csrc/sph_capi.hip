@@ -123,9 +123,17 @@ struct sph_ctx {
     bool use_graphs = false;
     hipGraphExec_t graph[2][2] = {};
 
+    // readback: pinned host buffers the device writes by id
+    float *rb_pos = nullptr, *rb_vel = nullptr;
+    size_t rb_cap = 0;
+    uint32_t *rb_bad = nullptr;
+
     // profiling
     hipEvent_t ev[kProfileSlots][2] = {};
     int ev_stage[kProfileSlots] = {};
+    uint64_t ev_step[kProfileSlots] = {};
+    uint32_t event_stride = 10;
+    std::vector<sph_profile_event> events;
     int ev_used = 0;
     float stage_ms[SPH_STAGE_COUNT] = {};
     uint32_t stage_launches[SPH_STAGE_COUNT] = {};
@@ -250,6 +258,8 @@ void release(sph_ctx *c)
     if (!c) return;
     if (c->stream) (void) hipStreamSynchronize(c->stream);
     drop_graphs(c);
+    if (c->rb_pos) (void) hipHostFree(c->rb_pos);
+    if (c->rb_vel) (void) hipHostFree(c->rb_vel);
     for (void *p : c->allocations) (void) hipFree(p);
     c->allocations.clear();
     for (auto &pair : c->ev)
@@ -294,8 +304,25 @@ sph_status drain_profile(sph_ctx *c)
     for (int i = 0; i < c->ev_used; ++i) {
         float ms = 0.0f;
         SPH_HIP(hipEventElapsedTime(&ms, c->ev[i][0], c->ev[i][1]));
-        c->stage_ms[c->ev_stage[i]] += ms;
-        c->stage_launches[c->ev_stage[i]] += 1;
+        const int stage = c->ev_stage[i];
+        c->stage_ms[stage] += ms;
+        c->stage_launches[stage] += 1;
+        const uint64_t step = c->ev_step[i];
+        if (c->event_stride && step % c->event_stride == 0) {
+            if (c->events.empty() || c->events.back().iteration != step) {
+                sph_profile_event e{};
+                e.iteration = step;
+                c->events.push_back(e);
+            }
+            sph_profile_event &e = c->events.back();
+            switch (stage) {
+            case SPH_STAGE_DENSITY: e.update_density_pressure += ms; break;
+            case SPH_STAGE_FORCES: e.update_forces += ms; break;
+            case SPH_STAGE_COLLISIONS: e.update_collisions += ms; break;
+            case SPH_STAGE_INTEGRATE: e.integrate += ms; break;
+            default: e.update_grid += ms; break;   // hash, sort, cells, reorder
+            }
+        }
     }
     c->ev_used = 0;
     return SPH_OK;
@@ -312,6 +339,7 @@ struct StageTimer {
         if (st != SPH_OK) return;
         slot = c->ev_used++;
         c->ev_stage[slot] = stage;
+        c->ev_step[slot] = c->steps_done;
         (void) hipEventRecord(c->ev[slot][0], c->stream);
     }
     ~StageTimer()
@@ -407,7 +435,8 @@ sph_status rebuild_layout(sph_ctx *c)
     c->total_cells = (uint32_t) coff;
     c->nslots = (uint32_t) active;
 
-    const uint32_t ngroups = (c->pool + sph::kGroupSlots - 1) / sph::kGroupSlots;
+    // only the groups below the active slots are ever read
+    const uint32_t ngroups = (uint32_t) (((uint64_t) c->nslots + sph::kGroupSlots - 1) / sph::kGroupSlots);
     c->group_h.assign(ngroups, 0);
     size_t t = 0;
     for (uint32_t g = 0; g < ngroups; ++g) {
@@ -423,7 +452,9 @@ sph_status rebuild_layout(sph_ctx *c)
         c->cstart_cap = want;
     }
     SPH_HIP(hipMemcpy(c->table_d, c->table_h.data(), c->table_h.size() * sizeof(sph::Scene), hipMemcpyHostToDevice));
-    SPH_HIP(hipMemcpy(c->group_d, c->group_h.data(), c->group_h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!c->group_h.empty())
+        SPH_HIP(hipMemcpy(c->group_d, c->group_h.data(), c->group_h.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
     if (!c->emit_h.empty())
         SPH_HIP(hipMemcpy(c->emit_d, c->emit_h.data(), c->emit_h.size() * sizeof(sph::Emitter),
                           hipMemcpyHostToDevice));
@@ -1428,6 +1459,99 @@ sph_status sph_autotune(sph_ctx *c, uint32_t steps, float *ms_out)
     c->use_lists = best_l;
     drop_graphs(c);
     return st ? st : st2;
+}
+
+sph_status sph_scene_readback(sph_ctx *c, uint32_t scene, int with_velocity, const float **pos_xyz,
+                              const float **vel_xyz, uint32_t *count)
+{
+    SPH_TRY(check_scene(c, scene, "sph_scene_readback"));
+    if (!pos_xyz || !count) return fail(SPH_ERR_INVALID, "sph_scene_readback: null output");
+    SPH_TRY(sync_counts(c));
+    const sph::Scene &s = c->scenes[scene].s;
+    const uint32_t n = s.n;
+    if (n > c->rb_cap || (with_velocity && !c->rb_vel)) {
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        const size_t cap = std::max<size_t>(n, c->rb_cap);
+        if (c->rb_pos) (void) hipHostFree(c->rb_pos);
+        if (c->rb_vel) (void) hipHostFree(c->rb_vel);
+        c->rb_pos = c->rb_vel = nullptr;
+        c->rb_cap = 0;
+        SPH_HIP(hipHostMalloc((void **) &c->rb_pos, std::max<size_t>(cap, 1) * 12, hipHostMallocDefault));
+        if (with_velocity) SPH_HIP(hipHostMalloc((void **) &c->rb_vel, std::max<size_t>(cap, 1) * 12, hipHostMallocDefault));
+        c->rb_cap = cap;
+    }
+    if (!c->rb_bad) SPH_TRY(dalloc(c, &c->rb_bad, 1));
+    SPH_HIP(hipMemsetAsync(c->rb_bad, 0, 4, c->stream));
+    sph::launch_scatter_by_id(c->soa(c->cur, c->id_cur), s.poff, n, c->rb_pos, with_velocity ? c->rb_vel : nullptr,
+                              c->rb_bad, c->stream);
+    SPH_TRY(launch_error("readback launch"));
+    uint32_t bad = 0;
+    SPH_TRY(copy_d2h(c, &bad, c->rb_bad, 4));
+    if (bad) return fail(SPH_ERR_STATE, "sph_scene_readback: particle ids are not 0..count-1");
+    *pos_xyz = c->rb_pos;
+    if (vel_xyz) *vel_xyz = with_velocity ? c->rb_vel : nullptr;
+    *count = n;
+    return SPH_OK;
+}
+
+sph_status sph_set_event_stride(sph_ctx *c, uint32_t stride)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    c->event_stride = stride;
+    return SPH_OK;
+}
+
+sph_status sph_profile_events(sph_ctx *c, sph_profile_event *out, uint32_t cap, uint32_t *n, int reset)
+{
+    if (!check_ctx(c)) return SPH_ERR_INVALID;
+    if (!(c->flags & SPH_FLAG_PROFILE)) return fail(SPH_ERR_STATE, "sph_profile_events: needs SPH_FLAG_PROFILE");
+    SPH_TRY(drain_profile(c));
+    if (out)
+        for (size_t i = 0; i < c->events.size() && i < cap; ++i) out[i] = c->events[i];
+    if (n) *n = (uint32_t) c->events.size();
+    if (reset) c->events.clear();
+    return SPH_OK;
+}
+
+sph_status sph_write_logs(const sph_profile_event *ev, uint32_t n, const char *stem, const char *label)
+{
+    if (!stem || !label || (n && !ev)) return fail(SPH_ERR_INVALID, "sph_write_logs: null argument");
+    const std::string base(stem);
+    FILE *sum = std::fopen((base + ".csv").c_str(), "a");
+    if (!sum) return fail(SPH_ERR_INVALID, "sph_write_logs: cannot open " + base + ".csv");
+    FILE *det = std::fopen((base + "_detail.csv").c_str(), "a");
+    if (!det) {
+        std::fclose(sum);
+        return fail(SPH_ERR_INVALID, "sph_write_logs: cannot open " + base + "_detail.csv");
+    }
+    // summary: the label, then every event's together() (src/mainwindow.cpp:329-345)
+    std::fprintf(sum, "%s;", label);
+    for (uint32_t i = 0; i < n; ++i)
+        std::fprintf(sum, "%g;", ev[i].update_grid + ev[i].update_density_pressure + ev[i].update_forces +
+                                      ev[i].update_collisions + ev[i].integrate);
+    std::fprintf(sum, "\n");
+    // detail: the label, one line per phase, two empty lines (src/mainwindow.cpp:347-362)
+    std::fprintf(det, "%s\n", label);
+    const char *names[5] = {"Grid", "Density + pressure", "Forces", "Collisions", "Integrate"};
+    for (int ph = 0; ph < 5; ++ph) {
+        std::fprintf(det, "%s;", names[ph]);
+        for (uint32_t i = 0; i < n; ++i) {
+            const double v = ph == 0 ? ev[i].update_grid : ph == 1 ? ev[i].update_density_pressure
+                           : ph == 2 ? ev[i].update_forces : ph == 3 ? ev[i].update_collisions : ev[i].integrate;
+            std::fprintf(det, "%g;", v);
+        }
+        std::fprintf(det, "\n");
+    }
+    std::fprintf(det, "\n\n");
+    const bool ok = std::fclose(sum) == 0 && std::fclose(det) == 0;
+    return ok ? SPH_OK : fail(SPH_ERR_INVALID, "sph_write_logs: write failed");
+}
+
+sph_status sph_export_logs(sph_ctx *c, const char *stem, const char *label)
+{
+    uint32_t n = 0;
+    SPH_TRY(sph_profile_events(c, nullptr, 0, &n, 0));
+    return sph_write_logs(c->events.data(), (uint32_t) c->events.size(), stem, label);
 }
 
 sph_status sph_device_name(char *buf, size_t len)

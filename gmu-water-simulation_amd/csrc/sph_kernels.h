@@ -83,6 +83,10 @@ void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells,
 void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32_t *perm, uint32_t n,
                     hipStream_t st);
 
+// pos[3*id + c] (and vel) for slots first .. first+n-1; ids outside 0..n-1 set *bad.
+void launch_scatter_by_id(const ParticleSoA &p, uint32_t first, uint32_t n, float *pos, float *vel, uint32_t *bad,
+                          hipStream_t st);
+
 // Density pass: density, p/rho^2, 1/rho and the neighbour lists.
 void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
                     const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, hipStream_t st);

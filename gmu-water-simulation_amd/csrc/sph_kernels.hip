@@ -388,6 +388,31 @@ __global__ void k_reorder(ParticleSoA src, ParticleSoA dst, const uint32_t *__re
     dst.id[i] = src.id[j];
 }
 
+// Particles first .. first+n-1 of the current order written by id into xyz
+// triples (render/readback order, as CGPUBaseParticleSimulator::integrate
+// un-permutes by id, src/CGPUBaseParticleSimulator.cpp:83-91). vel may be null.
+// An id outside 0..n-1 is not written; it raises *bad instead.
+__global__ __launch_bounds__(256) void k_scatter_by_id(ParticleSoA p, uint32_t first, uint32_t n,
+                                                       float *__restrict__ pos, float *__restrict__ vel,
+                                                       uint32_t *__restrict__ bad)
+{
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = first + k, id = p.id[i];
+    if (id >= n) {
+        atomicOr(bad, 1u);
+        return;
+    }
+    pos[3 * (size_t) id + 0] = p.px[i];
+    pos[3 * (size_t) id + 1] = p.py[i];
+    pos[3 * (size_t) id + 2] = p.pz[i];
+    if (vel) {
+        vel[3 * (size_t) id + 0] = p.vx[i];
+        vel[3 * (size_t) id + 1] = p.vy[i];
+        vel[3 * (size_t) id + 2] = p.vz[i];
+    }
+}
+
 // ---------------------------------------------------------------- neighbours
 //
 // Canonical neighbour order: rows z-1..z+1 (outer), y-1..y+1, and within a row
@@ -846,6 +871,13 @@ void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleS
 {
     if (nem == 0) return;
     hipLaunchKernelGGL(k_emit, dim3(nem), dim3(256), 0, st, em, scenes, p, key, h);
+}
+
+void launch_scatter_by_id(const ParticleSoA &p, uint32_t first, uint32_t n, float *pos, float *vel, uint32_t *bad,
+                          hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_scatter_by_id, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, first, n, pos, vel, bad);
 }
 
 void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, double h, hipStream_t st)

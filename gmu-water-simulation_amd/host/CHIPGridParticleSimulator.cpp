@@ -115,19 +115,19 @@ double CHIPGridParticleSimulator::integrate()
 {
     const double ms = phase(SPH_PHASE_INTEGRATE);
     if (m_failed) return ms;
-    // readback by id for rendering (src/CGPUBaseParticleSimulator.cpp:83-91)
-    const uint32_t n = m_onDevice;
-    m_pos.resize(3 * (size_t) n);
-    m_vel.resize(3 * (size_t) n);
-    if (!ok(sph_download(m_ctx, m_pos.data(), m_vel.data(), nullptr, n, 1))) return ms;
+    // readback by id for rendering (src/CGPUBaseParticleSimulator.cpp:83-91): the device
+    // writes positions and velocities by id into the library's pinned buffers
+    const float *pos = nullptr, *vel = nullptr;
+    uint32_t n = 0;
+    if (!ok(sph_scene_readback(m_ctx, 0, 1, &pos, &vel, &n))) return ms;
     for (uint32_t i = 0; i < n; ++i) {
         CParticle::Physics &p = m_clParticles[i];
-        p.position.s[0] = m_pos[3 * i];
-        p.position.s[1] = m_pos[3 * i + 1];
-        p.position.s[2] = m_pos[3 * i + 2];
-        p.velocity.s[0] = m_vel[3 * i];
-        p.velocity.s[1] = m_vel[3 * i + 1];
-        p.velocity.s[2] = m_vel[3 * i + 2];
+        p.position.s[0] = pos[3 * i];
+        p.position.s[1] = pos[3 * i + 1];
+        p.position.s[2] = pos[3 * i + 2];
+        p.velocity.s[0] = vel[3 * i];
+        p.velocity.s[1] = vel[3 * i + 1];
+        p.velocity.s[2] = vel[3 * i + 2];
     }
     for (auto &particle : m_grid->getData()[0]) {
         particle->updatePosition();

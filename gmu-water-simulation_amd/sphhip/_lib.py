@@ -69,6 +69,11 @@ SIGNATURES = {
     "sph_variant_shape": (C.c_int, [C.c_int, _vp, _vp]),
     "sph_autotune": (C.c_int, [_vp, _u32, _vp]),
     "sph_slot_bytes": (C.c_uint64, [_u32]),
+    "sph_scene_readback": (C.c_int, [_vp, _u32, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_u32)]),
+    "sph_set_event_stride": (C.c_int, [_vp, _u32]),
+    "sph_profile_events": (C.c_int, [_vp, _vp, _u32, C.POINTER(_u32), C.c_int]),
+    "sph_write_logs": (C.c_int, [_vp, _u32, C.c_char_p, C.c_char_p]),
+    "sph_export_logs": (C.c_int, [_vp, C.c_char_p, C.c_char_p]),
     "sph_server_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "sph_server_destroy": (None, [_vp]),
     "sph_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
@@ -96,6 +101,12 @@ class sph_batch_params(C.Structure):
 class sph_emitter(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("rate", C.c_uint32), ("limit", C.c_uint32), ("centre", C.c_float * 2),
                 ("radius", C.c_float), ("speed", C.c_float), ("spread", C.c_float), ("seed", C.c_uint64)]
+
+
+class sph_profile_event(C.Structure):
+    _fields_ = [("iteration", C.c_uint64), ("fps", C.c_double), ("update_grid", C.c_double),
+                ("update_density_pressure", C.c_double), ("update_forces", C.c_double),
+                ("update_collisions", C.c_double), ("integrate", C.c_double)]
 
 
 class SphError(RuntimeError):
@@ -136,6 +147,37 @@ def ptr(a):
 def f32(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a if shape is None else a.reshape(shape)
+
+
+def readback(lib, ctx, scene, with_velocity=True):
+    """sph_scene_readback: positions (and velocities) by id, copied out of the
+    library's pinned buffer."""
+    pp, vp, n = C.c_void_p(), C.c_void_p(), C.c_uint32(0)
+    check(lib.sph_scene_readback(ctx, int(scene), int(bool(with_velocity)), C.byref(pp), C.byref(vp), C.byref(n)))
+    k = n.value
+    if k == 0:
+        return np.zeros((0, 3), np.float32), (np.zeros((0, 3), np.float32) if with_velocity else None)
+    pos = np.ctypeslib.as_array(C.cast(pp, C.POINTER(C.c_float)), shape=(k, 3)).copy()
+    vel = np.ctypeslib.as_array(C.cast(vp, C.POINTER(C.c_float)), shape=(k, 3)).copy() if with_velocity else None
+    return pos, vel
+
+
+def profile_events(lib, ctx, reset=False):
+    """sph_profile_events as a list of dicts (sProfilingEvent fields)."""
+    n = C.c_uint32(0)
+    check(lib.sph_profile_events(ctx, None, 0, C.byref(n), 0))
+    arr = (sph_profile_event * max(n.value, 1))()
+    check(lib.sph_profile_events(ctx, arr, n.value, C.byref(n), int(bool(reset))))
+    return [{f: getattr(arr[i], f) for f, _ in sph_profile_event._fields_} for i in range(n.value)]
+
+
+def write_logs(events, stem, label):
+    """sph_write_logs for a list of event dicts (exportLogs format)."""
+    arr = (sph_profile_event * max(len(events), 1))()
+    for i, e in enumerate(events):
+        for f, _ in sph_profile_event._fields_:
+            setattr(arr[i], f, e.get(f, 0))
+    check(load().sph_write_logs(arr, len(events), str(stem).encode(), str(label).encode()))
 
 
 def u32(a):

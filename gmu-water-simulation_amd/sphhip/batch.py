@@ -107,6 +107,10 @@ class SceneBatch:
                                                 n, int(by_id)))
         return pos, vel, ids
 
+    def readback(self, scene, with_velocity=True):
+        """Positions (and velocities) of a scene by id via the device scatter into pinned memory."""
+        return _lib.readback(self._lib, self._ctx, scene, with_velocity)
+
     def set_gravity(self, scene, g):
         g = np.asarray(g, np.float32).reshape(3)
         _lib.check(self._lib.sph_scene_set_gravity(self._ctx, int(scene), *[float(v) for v in g]))

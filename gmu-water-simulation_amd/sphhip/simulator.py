@@ -224,6 +224,19 @@ class HIPGridParticleSimulator:
         _lib.check(self._lib.sph_download(self._ctx, _lib.ptr(pos), _lib.ptr(vel), _lib.ptr(ids), n, int(by_id)))
         return pos, vel, ids
 
+    def readback(self, with_velocity=True):
+        """Positions (and velocities) by id, written by the device into pinned
+        host memory (the render boundary of integrate())."""
+        return _lib.readback(self._lib, self._ctx, 0, with_velocity)
+
+    def profile_events(self, reset=False):
+        """Logged sProfilingEvents (needs FLAG_PROFILE)."""
+        return _lib.profile_events(self._lib, self._ctx, reset)
+
+    def exportLogs(self, stem, label="HIP Grid"):
+        """MainWindow::exportLogs format (src/mainwindow.cpp:310-368)."""
+        _lib.check(self._lib.sph_export_logs(self._ctx, str(stem).encode(), str(label).encode()))
+
     def grid_dims(self):
         xyz = np.zeros(3, np.int32)
         nc = C.c_uint32(0)

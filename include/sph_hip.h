@@ -292,6 +292,52 @@ sph_status sph_variant_shape(int variant, int *threads, int *unroll);
  * negative where a choice was not run. */
 sph_status sph_autotune(sph_ctx *ctx, uint32_t steps, float *ms_out);
 
+/* ------------------------------------------------------------------------------
+ * Render/readback boundary and profiling export.
+ * ------------------------------------------------------------------------------ */
+
+/* Replaces: the blocking readback and CPU un-permutation of
+ * CGPUBaseParticleSimulator::integrate (src/CGPUBaseParticleSimulator.cpp:76-97). The
+ * device writes the scene's positions (and velocities if with_velocity) by particle id,
+ * xyz triples, straight into library-owned pinned host memory (no staging copies);
+ * *pos_xyz / *vel_xyz point to it until the next readback of the context or its
+ * destruction, *count = the scene's particle count. Ids must be 0..count-1
+ * (SPH_ERR_STATE otherwise). */
+sph_status sph_scene_readback(sph_ctx *ctx, uint32_t scene, int with_velocity, const float **pos_xyz,
+                              const float **vel_xyz, uint32_t *count);
+
+/* One sProfilingEvent (include/SProfilingEvent.h:8-30): device ms of each phase of
+ * one step. Fused steps report forces + collisions + integrate under update_forces
+ * (one kernel) and 0 for the other two. fps is not measured (the library has no frame
+ * loop) and is 0. */
+typedef struct {
+    uint64_t iteration;
+    double fps;
+    double update_grid;
+    double update_density_pressure;
+    double update_forces;
+    double update_collisions;
+    double integrate;
+} sph_profile_event;
+
+/* With SPH_FLAG_PROFILE, every `stride`-th step (0 = none; default 10, the reference's
+ * eventLoggerStride, include/CBaseParticleSimulator.h:55) is logged as one event. */
+sph_status sph_set_event_stride(sph_ctx *ctx, uint32_t stride);
+
+/* Copies up to cap logged events (out may be NULL) and stores the total in *n; reset
+ * clears the log. Needs SPH_FLAG_PROFILE. */
+sph_status sph_profile_events(sph_ctx *ctx, sph_profile_event *out, uint32_t cap, uint32_t *n, int reset);
+
+/* Replaces: MainWindow::exportLogs (src/mainwindow.cpp:310-368). Appends the events in
+ * its format to <stem>.csv ("label;together;together;...") and <stem>_detail.csv (the
+ * label, then one line per phase "Grid;..", "Density + pressure;..", "Forces;..",
+ * "Collisions;..", "Integrate;..", then two empty lines); numbers as QString::number
+ * (printf %g). No context needed. */
+sph_status sph_write_logs(const sph_profile_event *events, uint32_t n, const char *stem, const char *label);
+
+/* sph_profile_events + sph_write_logs for a context. */
+sph_status sph_export_logs(sph_ctx *ctx, const char *stem, const char *label);
+
 /* Device bytes one particle slot of a context costs (particle state, sort buffers,
  * density fields, records, lists of max_neighbours entries, its share of the cell
  * table), for sizing a pool against an HBM budget. */

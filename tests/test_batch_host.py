@@ -69,3 +69,22 @@ def test_disc_emission_is_keyed_by_particle(orc):
     b1, vb1 = orc.disc_emission(9, 0, 10, (0, 0), 0.2, 1.0, 0.1, np.float32(-0.5))
     b2, vb2 = orc.disc_emission(9, 10, 15, (0, 0), 0.2, 1.0, 0.1, np.float32(-0.5))
     assert np.array_equal(a, np.vstack([b1, b2])) and np.array_equal(va, np.vstack([vb1, vb2]))
+
+
+def test_write_logs_matches_export_format(hip, tmp_path):
+    """sph_write_logs appends the two files of MainWindow::exportLogs
+    (src/mainwindow.cpp:310-368): summary 'label;together;...' and the detail
+    block (label, one line per phase, two empty lines), numbers as %g."""
+    ev = [dict(iteration=0, update_grid=0.125, update_density_pressure=1.5, update_forces=2.25,
+               update_collisions=0.0, integrate=0.0),
+          dict(iteration=10, update_grid=0.1, update_density_pressure=1234567.0, update_forces=3e-7,
+               update_collisions=0.5, integrate=0.25)]
+    stem = tmp_path / "Dam break_0.4"
+    hip._lib.write_logs(ev, stem, "HIP Grid")
+    hip._lib.write_logs(ev[:1], stem, "HIP Grid")          # appends, as the reference does
+    summary = (tmp_path / "Dam break_0.4.csv").read_text()
+    detail = (tmp_path / "Dam break_0.4_detail.csv").read_text()
+    assert summary == "HIP Grid;3.875;1.23457e+06;\nHIP Grid;3.875;\n"
+    assert detail == ("HIP Grid\nGrid;0.125;0.1;\nDensity + pressure;1.5;1.23457e+06;\nForces;2.25;3e-07;\n"
+                      "Collisions;0;0.5;\nIntegrate;0;0.25;\n\n\n"
+                      "HIP Grid\nGrid;0.125;\nDensity + pressure;1.5;\nForces;2.25;\nCollisions;0;\nIntegrate;0;\n\n\n")

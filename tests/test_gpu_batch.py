@@ -290,3 +290,57 @@ def test_batch_errors(hip):
             b.download(1)                   # no scene 1
         with pytest.raises(hip.SphError):
             b.set_emitter(0, kind=7)
+
+
+def test_readback_by_id_equals_download(hip):
+    scs = small_scenes(hip, count=3, seed=31)
+    with make_batch(hip, scs) as b:
+        b.step(6)
+        for i in range(len(scs)):
+            p, v, _ = b.download(i)
+            rp, rv = b.readback(i)
+            assert_bits_equal(rp, p, "readback positions scene %d" % i)
+            assert_bits_equal(rv, v, "readback velocities scene %d" % i)
+    pos = hip.scenes.lattice(8, 8, 8, 0.3, seed=2)
+    sim = hip.HIPGridParticleSimulator(0.3, capacity=len(pos))
+    sim.upload(pos, ids=np.arange(len(pos))[::-1].copy())     # ids in reverse order
+    sim.advance(3)
+    p, v, _ = sim.download()
+    rp, rv = sim.readback()
+    assert_bits_equal(rp, p, "single-scene readback")
+    sim.close()
+
+
+def test_readback_rejects_bad_ids(hip):
+    pos = hip.scenes.lattice(4, 4, 4, 0.3, seed=2)
+    sim = hip.HIPGridParticleSimulator(0.3, capacity=len(pos))
+    ids = np.arange(len(pos), dtype=np.uint32)
+    ids[5] = 1000                                            # outside 0..n-1: not written, reported
+    sim.upload(pos, ids=ids)
+    with pytest.raises(hip.SphError):
+        sim.readback()
+    sim.close()
+
+
+def test_profile_events_and_export(hip, tmp_path):
+    lib = hip._lib
+    pos = hip.scenes.lattice(12, 12, 12, 0.4, seed=3)
+    for fused in (True, False):
+        sim = hip.HIPGridParticleSimulator(0.4, capacity=len(pos), flags=lib.FLAG_PROFILE, fused=fused)
+        sim.upload(pos)
+        for _ in range(25):
+            sim.step()
+        ev = sim.profile_events()
+        assert [e["iteration"] for e in ev] == [0, 10, 20]
+        for e in ev:
+            assert e["update_grid"] > 0 and e["update_density_pressure"] > 0 and e["update_forces"] > 0
+            if fused:
+                assert e["update_collisions"] == 0 and e["integrate"] == 0
+            else:
+                assert e["update_collisions"] > 0 and e["integrate"] > 0
+        stem = tmp_path / ("fused" if fused else "phases")
+        sim.exportLogs(stem, "HIP Grid")
+        rows = (tmp_path / (stem.name + "_detail.csv")).read_text().splitlines()
+        assert rows[0] == "HIP Grid" and rows[1].startswith("Grid;") and rows[5].startswith("Integrate;")
+        assert len(rows[2].rstrip(";").split(";")) == 4                   # label + 3 events
+        sim.close()
