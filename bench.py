@@ -116,7 +116,7 @@ def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
         b.upload(i, s["pos"])
     b.step(2)
     tune = b.autotune(steps=3)
-    variant, lists = b.variant()
+    variant, mode = b.variant()
     b.set_graphs(True)
     b.sync()
     t0 = time.perf_counter()
@@ -128,8 +128,9 @@ def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
                        "boxes at 1/4 fill with per-axis aspect U[0.5, 2]" % (nscenes, seed),
            "scenes": nscenes, "particles": total, "steps": steps, "value": total * steps / wall,
            "unit": "particle-steps/s", "ms_per_step": wall * 1e3 / steps, "graphs": True,
-           "variant": {"index": variant, "threads_unroll": sphhip.variant_shape(variant), "lists": lists},
-           "autotune_ms_per_step": {"%d%s" % (v, "L" if l else ""): round(ms, 4) for (v, l), ms in tune.items()}}
+           "variant": {"index": variant, "threads_unroll": sphhip.variant_shape(variant),
+                       "neighbours": ("walk", "lists", "masks")[mode]},
+           "autotune_ms_per_step": {"%d%s" % (v, "wlm"[m]): round(ms, 4) for (v, m), ms in tune.items()}}
     if per_scene_steps:
         sims = []
         for s in w:

@@ -108,10 +108,11 @@ struct sph_ctx {
     float4 *rec = nullptr;
     float *acc[3] = {};
     uint32_t *ncount = nullptr;
-    uint32_t *nlist = nullptr;   // neighbour lists, max_neighbours x pool
+    uint32_t *nlist = nullptr;   // neighbour lists, max_neighbours per slot, wave-blocked
     uint32_t *nlist_cnt = nullptr;
     uint32_t kmax = 0;           // list capacity allocated
-    bool use_lists = false;
+    uint64_t *masks = nullptr;   // row masks (mode SPH_NEIGHBOURS_MASKS), allocated on first use
+    int nb_mode = SPH_NEIGHBOURS_WALK;
     int variant = sph::kDefaultVariant;
 
     bool keys_valid = false;   // key_h holds the cells of the current positions
@@ -157,8 +158,9 @@ struct sph_ctx {
         sph::NeighbourList nl;
         nl.idx = nlist;
         nl.cnt = nlist_cnt;
-        nl.kmax = use_lists ? kmax : 0u;
-        nl.stride = pool;
+        nl.kmax = kmax;
+        nl.mode = nb_mode;
+        nl.masks = masks;
         return nl;
     }
 
@@ -916,7 +918,7 @@ sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
     for (int a = 0; a < 3; ++a) SPH_TRY(dalloc(c, &c->acc[a], P));
     SPH_TRY(dalloc(c, &c->ncount, P));
     SPH_TRY(dalloc(c, &c->nlist_cnt, P));
-    SPH_TRY(dalloc(c, &c->nlist, (size_t) c->kmax * P));
+    SPH_TRY(dalloc(c, &c->nlist, (size_t) c->kmax * round_up(c->pool, 64)));
     SPH_TRY(dalloc(c, &c->table_d, max_segments));
     SPH_TRY(dalloc(c, &c->group_d, (P + sph::kGroupSlots - 1) / sph::kGroupSlots));
     SPH_TRY(dalloc(c, &c->emit_d, c->scenes.size()));
@@ -968,7 +970,8 @@ sph_status create_ctx(uint32_t pool, uint32_t max_scenes, uint32_t flags, uint32
     c->device = dev;
     c->pool = pool;
     c->kmax = max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0u : max_neighbours;
-    c->use_lists = c->kmax > 0;
+    // lists when asked for, else the row masks (the fastest measured)
+    c->nb_mode = c->kmax > 0 ? SPH_NEIGHBOURS_LISTS : SPH_NEIGHBOURS_MASKS;
     c->k = sph::make_consts();
     c->scenes.resize(max_scenes);
     for (uint32_t i = 0; i < nscenes; ++i) {
@@ -984,6 +987,7 @@ sph_status create_ctx(uint32_t pool, uint32_t max_scenes, uint32_t flags, uint32
         return fail(SPH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     sph_status st = alloc_buffers(c, 2 * max_scenes + 1);
+    if (!st && c->nb_mode == SPH_NEIGHBOURS_MASKS) st = dalloc(c, &c->masks, (size_t) 9 * 4 * round_up(c->pool, 64));
     if (!st) st = rebuild_layout(c);
     if (st) {
         std::string msg = g_last_error;
@@ -1087,9 +1091,10 @@ uint64_t sph_slot_bytes(uint32_t max_neighbours)
 {
     const uint64_t lists = max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0 : max_neighbours;
     // pos/vel ping-pong 48, ids 8, keys and values 20, rho 4, p/rho^2 8, 1/rho 8,
-    // records 64, accelerations 12, counts 8, list entries, and 4 for the
-    // histograms, the group map and the scene's share of the cell table
-    return 48 + 8 + 20 + 4 + 8 + 8 + 64 + 12 + 8 + 4 * lists + 4;
+    // records 64, accelerations 12, counts 8, row masks 288 (the default mode
+    // without lists), list entries, and 4 for the histograms, the group map and
+    // the scene's share of the cell table
+    return 48 + 8 + 20 + 4 + 8 + 8 + 64 + 12 + 8 + (lists ? 4 * lists : 288) + 4;
 }
 
 int sph_grid_res(float box)
@@ -1385,26 +1390,37 @@ sph_status sph_variant_shape(int variant, int *threads, int *unroll)
     return SPH_OK;
 }
 
-sph_status sph_set_variant(sph_ctx *c, int variant, int lists)
+// Allocates the row masks the first time the mode needs them.
+sph_status ensure_masks(sph_ctx *c)
+{
+    if (c->masks) return SPH_OK;
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    return dalloc(c, &c->masks, (size_t) 9 * 4 * round_up(c->pool, 64));
+}
+
+sph_status sph_set_variant(sph_ctx *c, int variant, int mode)
 {
     if (!check_ctx(c)) return SPH_ERR_INVALID;
     if (variant < 0 || variant >= sph::kNumVariants)
         return fail(SPH_ERR_INVALID, "sph_set_variant: no variant " + std::to_string(variant));
-    if (lists && c->kmax == 0)
+    if (mode < SPH_NEIGHBOURS_WALK || mode > SPH_NEIGHBOURS_MASKS)
+        return fail(SPH_ERR_INVALID, "sph_set_variant: no neighbour mode " + std::to_string(mode));
+    if (mode == SPH_NEIGHBOURS_LISTS && c->kmax == 0)
         return fail(SPH_ERR_STATE, "sph_set_variant: the context was created without neighbour lists");
     SPH_TRY(between_steps(c, "sph_set_variant"));
+    if (mode == SPH_NEIGHBOURS_MASKS) SPH_TRY(ensure_masks(c));
     SPH_HIP(hipStreamSynchronize(c->stream));
     c->variant = variant;
-    c->use_lists = lists != 0;
+    c->nb_mode = mode;
     drop_graphs(c);
     return SPH_OK;
 }
 
-sph_status sph_get_variant(const sph_ctx *c, int *variant, int *lists)
+sph_status sph_get_variant(const sph_ctx *c, int *variant, int *mode)
 {
     if (!check_ctx(c)) return SPH_ERR_INVALID;
     if (variant) *variant = c->variant;
-    if (lists) *lists = c->use_lists ? 1 : 0;
+    if (mode) *mode = c->nb_mode;
     return SPH_OK;
 }
 
@@ -1414,24 +1430,25 @@ sph_status sph_autotune(sph_ctx *c, uint32_t steps, float *ms_out)
     if (steps == 0) return fail(SPH_ERR_INVALID, "sph_autotune: steps must be > 0");
     SPH_TRY(between_steps(c, "sph_autotune"));
     SPH_TRY(check_runnable(c, "sph_autotune"));
+    SPH_TRY(ensure_masks(c));
+    constexpr int kModes = 3;
     if (ms_out)
-        for (int i = 0; i < 2 * sph::kNumVariants; ++i) ms_out[i] = -1.0f;
+        for (int i = 0; i < kModes * sph::kNumVariants; ++i) ms_out[i] = -1.0f;
     const uint32_t saved_flags = c->flags;
     const long saved_fault = c->fault_at_step;
     c->flags &= ~(uint32_t) SPH_FLAG_PROFILE;
     c->fault_at_step = -1;
     Snapshot snap{c};
     sph_status st = snap.save();
-    int best_v = c->variant;
-    bool best_l = c->use_lists;
+    int best_v = c->variant, best_m = c->nb_mode;
     float best = -1.0f;
-    for (int lists = 0; lists < 2 && !st; ++lists) {
-        if (lists && c->kmax == 0) break;
+    for (int mode = 0; mode < kModes && !st; ++mode) {
+        if (mode == SPH_NEIGHBOURS_LISTS && c->kmax == 0) continue;
         for (int v = 0; v < sph::kNumVariants && !st; ++v) {
             st = snap.restore();
             if (st) break;
             c->variant = v;
-            c->use_lists = lists != 0;
+            c->nb_mode = mode;
             drop_graphs(c);
             st = run_step(c);   // warm-up (and graph capture)
             if (!st) st = run_step(c);
@@ -1444,11 +1461,11 @@ sph_status sph_autotune(sph_ctx *c, uint32_t steps, float *ms_out)
                 st = fail(SPH_ERR_HIP, "hipEventElapsedTime");
             if (st) break;
             ms /= (float) steps;
-            if (ms_out) ms_out[2 * v + lists] = ms;
+            if (ms_out) ms_out[kModes * v + mode] = ms;
             if (best < 0.0f || ms < best) {
                 best = ms;
                 best_v = v;
-                best_l = lists != 0;
+                best_m = mode;
             }
         }
     }
@@ -1456,7 +1473,7 @@ sph_status sph_autotune(sph_ctx *c, uint32_t steps, float *ms_out)
     c->flags = saved_flags;
     c->fault_at_step = saved_fault;
     c->variant = best_v;
-    c->use_lists = best_l;
+    c->nb_mode = best_m;
     drop_graphs(c);
     return st ? st : st2;
 }

@@ -27,17 +27,30 @@ struct DensityFields {
     float4 *rec;
 };
 
-// Accepted neighbours of every particle except itself, in canonical order,
-// stored neighbour-major: idx[k * stride + i] is the k-th neighbour of slot i,
-// as a slot relative to the first slot of i's scene.
-// cnt[i] is the true count; particles with cnt > kmax have no usable list and
-// the force pass re-walks their cells.
+// Accepted neighbours of every particle except itself, in canonical order, as
+// slots relative to the first slot of the particle's scene. Blocked by wave:
+// the k-th neighbour of slot i is idx[((i / 64) * kmax + k) * 64 + i % 64], so
+// that the 64 particles of a wave keep their lists in one kmax x 64 block and a
+// wave reads one level of all its lists with one coalesced load. cnt[i] is the
+// true count; particles with cnt > kmax have no usable list and the force pass
+// re-walks their cells.
+//
+// Or (mode kNbMasks) the density pass's per-row accepted-candidate masks
+// (`masks`, see the kernels), with cnt[i] = 1 for a particle they cannot hold.
+enum NeighbourMode : int { kNbWalk = 0, kNbLists = 1, kNbMasks = 2 };
+
 struct NeighbourList {
     uint32_t *idx;
     uint32_t *cnt;
     uint32_t kmax;
-    uint32_t stride;
+    int mode;            // NeighbourMode
+    uint64_t *masks;
 };
+
+__host__ __device__ inline size_t list_slot(const NeighbourList &nl, uint32_t i)
+{
+    return (size_t) (i >> 6) * nl.kmax * 64 + (i & 63u);
+}
 
 constexpr int kRadixBits = 8;
 constexpr int kRadixBuckets = 1 << kRadixBits;
@@ -56,7 +69,7 @@ struct NeighbourVariant {
     int unroll;
 };
 constexpr int kNumVariants = 8;
-constexpr int kDefaultVariant = 2;   // 64 threads, 4 candidates: fastest measured on both workloads
+constexpr int kDefaultVariant = 7;   // 64 threads, 8 candidates per trip: fastest measured with row masks
 extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 
 // Emitters (one workgroup each) add particles at the start of a step and update

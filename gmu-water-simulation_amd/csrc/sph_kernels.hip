@@ -480,37 +480,74 @@ __device__ __forceinline__ FaceBounds face_bounds(float p, float box, int c, dou
     return b;
 }
 
-// Walks the particle's candidates in canonical order: rows (z-1..z+1 outer,
-// y-1..y+1), each a contiguous run of the cells x-1..x+1. Rows and end cells of
-// a run whose distance lower bound exceeds h (with a 1e-5 relative margin, far
+// Row r (0..8) of a particle's neighbourhood: z-1..z+1 (outer), y-1..y+1, and in
+// it the contiguous run of cells x-1..x+1 in slot order. Rows and end cells of a
+// run whose distance lower bound exceeds h (with a 1e-5 relative margin, far
 // above the float rounding of r^2) are left out: every candidate they hold would
 // fail the r^2 <= h^2 test, so the accepted neighbours and their order are
-// unchanged. test(j, valid, xj, yj, zj) sees U candidates per trip in slot order
-// (j relative to the scene's first slot) and appends without branching (writes
-// slot cnt, then adds the accept bit), so a list keeps U slots of headroom; when
-// any lane's list is that full, flush() drains every lane's list.
-template <int U, typename Test, typename Flush>
+// unchanged. The density and force passes compute the same runs from the same
+// inputs.
+struct RowBounds {
+    FaceBounds bx, by, bz;
+    double lim;
+};
+
+__device__ __forceinline__ RowBounds row_bounds(const Scene &s, const Consts &k, int x, int y, int z, float xi,
+                                                float yi, float zi)
+{
+    RowBounds rb;
+    rb.bx = face_bounds(xi, s.bx, x, k.h);
+    rb.by = face_bounds(yi, s.by, y, k.h);
+    rb.bz = face_bounds(zi, s.bz, z, k.h);
+    rb.lim = k.hsq * (1.0 + 1e-5);
+    return rb;
+}
+
+// [j, jend): the candidate slots of row r, relative to the scene's first slot
+// (empty when the row is off the grid or culled).
+__device__ __forceinline__ void row_run(const RowBounds &rb, const Scene &s, const uint32_t *__restrict__ start, int x,
+                                        int y, int z, int r, uint32_t &j, uint32_t &jend)
+{
+    const int oz = r / 3 - 1, oy = r % 3 - 1;
+    const int zz = z + oz, yy = y + oy;
+    const double lyz =
+        (oy < 0 ? rb.by.lo2 : oy > 0 ? rb.by.hi2 : 0.0) + (oz < 0 ? rb.bz.lo2 : oz > 0 ? rb.bz.hi2 : 0.0);
+    j = 0;
+    jend = 0;
+    if (zz >= 0 && zz < s.rz && yy >= 0 && yy < s.ry && lyz <= rb.lim) {
+        const uint32_t row = s.coff + (uint32_t) (yy * s.rx + zz * s.rx * s.ry);
+        const int x0 = (x > 0 && lyz + rb.bx.lo2 <= rb.lim) ? x - 1 : x;
+        const int x1 = (x + 1 < s.rx && lyz + rb.bx.hi2 <= rb.lim) ? x + 1 : x;
+        j = start[row + x0] - s.poff;
+        jend = start[row + x1 + 1] - s.poff;
+    }
+}
+
+// Hooks of walk_neighbours that record nothing.
+struct NoRowHooks {
+    __device__ __forceinline__ void row_begin(int, uint32_t, uint32_t) {}
+    __device__ __forceinline__ void trip(uint32_t, uint32_t) {}
+    __device__ __forceinline__ void row_end(int) {}
+};
+
+// Walks the particle's candidates in canonical order, row by row (row_run).
+// test(j, valid, xj, yj, zj) sees U candidates per trip in slot order (j
+// relative to the scene's first slot), returns whether it accepted the
+// candidate, and appends without branching (writes slot cnt, then adds the
+// accept bit), so a list keeps U slots of headroom; when any lane's list is that
+// full, flush() drains every lane's list. hooks.trip(j0, bits) gets the accept
+// bits of each trip (bit u: candidate j0 + u).
+template <int U, typename Test, typename Flush, typename Hooks = NoRowHooks>
 __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k, const uint32_t *__restrict__ start,
                                                 const BufView &bv, int x, int y, int z, float xi, float yi,
-                                                float zi, int &cnt, Test test, Flush flush)
+                                                float zi, int &cnt, Test test, Flush flush, Hooks &&hooks = Hooks())
 {
     static_assert(U < kListCap, "a list must hold one trip of candidates");
-    const FaceBounds bx = face_bounds(xi, s.bx, x, k.h);
-    const FaceBounds by = face_bounds(yi, s.by, y, k.h);
-    const FaceBounds bz = face_bounds(zi, s.bz, z, k.h);
-    const double lim = k.hsq * (1.0 + 1e-5);
+    const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
     for (int r = 0; r < 9; ++r) {
-        const int oz = r / 3 - 1, oy = r % 3 - 1;
-        const int zz = z + oz, yy = y + oy;
-        const double lyz = (oy < 0 ? by.lo2 : oy > 0 ? by.hi2 : 0.0) + (oz < 0 ? bz.lo2 : oz > 0 ? bz.hi2 : 0.0);
-        uint32_t j = 0, jend = 0;
-        if (zz >= 0 && zz < s.rz && yy >= 0 && yy < s.ry && lyz <= lim) {
-            const uint32_t row = s.coff + (uint32_t) (yy * s.rx + zz * s.rx * s.ry);
-            const int x0 = (x > 0 && lyz + bx.lo2 <= lim) ? x - 1 : x;
-            const int x1 = (x + 1 < s.rx && lyz + bx.hi2 <= lim) ? x + 1 : x;
-            j = start[row + x0] - s.poff;
-            jend = start[row + x1 + 1] - s.poff;
-        }
+        uint32_t j, jend;
+        row_run(rb, s, start, x, y, z, r, j, jend);
+        hooks.row_begin(r, j, jend);
         while (__any(j < jend)) {
             if (j < jend) {
                 const uint32_t off = j * 4u, rem = jend - j;
@@ -521,8 +558,11 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
                     cy[u] = ldf(bv.py, off + 4u * u);
                     cz[u] = ldf(bv.pz, off + 4u * u);
                 }
+                uint32_t bits = 0;
 #pragma unroll
-                for (int u = 0; u < U; ++u) test(j + u, (uint32_t) u < rem, cx[u], cy[u], cz[u]);
+                for (int u = 0; u < U; ++u)
+                    bits |= (uint32_t) test(j + u, (uint32_t) u < rem, cx[u], cy[u], cz[u]) << u;
+                hooks.trip(j, bits);
                 j += U;
             }
             if (__any(cnt > kListCap - U)) {
@@ -530,19 +570,72 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
                 cnt = 0;
             }
         }
+        hooks.row_end(r);
     }
     flush();
     cnt = 0;
 }
 
+// Accepted-candidate masks of the density pass, for the force pass: bit b of
+// row r's mask is candidate (run start + b) of that row. A row holds up to
+// kMaskBits candidates; a particle with a longer run has no masks (its force
+// pass walks). Stored word-major per wave: word w of row r of slot i is
+// masks[(((i / 64) * 9 + r) * kMaskWords + w) * 64 + i % 64], so the lanes of a
+// wave write and read each word with one coalesced access.
+constexpr uint32_t kMaskWords = 4;
+constexpr uint32_t kMaskBits = 64 * kMaskWords;
+
+__device__ __forceinline__ size_t mask_word(uint32_t i, int r, uint32_t w)
+{
+    return ((size_t) ((i >> 6) * 9u + (uint32_t) r) * kMaskWords + w) * 64u + (i & 63u);
+}
+
+// Records the accept bits of each trip into the row's current 64-bit word,
+// storing a word when the run moves past it and at the end of the row.
+struct MaskRecorder {
+    uint64_t *masks;
+    uint32_t i;
+    uint32_t jrow = 0, len = 0, word = 0;
+    uint64_t cur = 0;
+    bool overflow = false;
+    int row = 0;
+
+    __device__ __forceinline__ void row_begin(int r, uint32_t j, uint32_t jend)
+    {
+        row = r;
+        jrow = j;
+        len = jend - j;
+        word = 0;
+        cur = 0;
+        overflow |= len > kMaskBits;
+    }
+    __device__ __forceinline__ void trip(uint32_t j0, uint32_t bits)
+    {
+        const uint32_t b = j0 - jrow, w = b >> 6;
+        if (w != word) {   // runs visit their candidates in order: word only grows
+            if (word < kMaskWords) masks[mask_word(i, row, word)] = cur;
+            word = w;
+            cur = 0;
+        }
+        cur |= (uint64_t) bits << (b & 63u);
+    }
+    __device__ __forceinline__ void row_end(int)
+    {
+        if (len > 0 && word < kMaskWords) masks[mask_word(i, row, word)] = cur;
+    }
+};
+
 // updateDensityPressure, src/CCPUParticleSimulator.cpp:93-141. Also stores
 // p/rho^2 (double, as src/CCPUParticleSimulator.cpp:181 forms it), 1/rho
-// (double) and, with kLists, the particle's accepted neighbours (minus itself)
-// for the force pass. T threads per workgroup, U candidates per lane per trip.
-template <int T, int U, bool kLists>
+// (double) and, for the force pass, with kLists the particle's accepted
+// neighbours (minus itself), with kMasks its rows' accepted-candidate masks
+// (masks, and nl.cnt[i] = 1 when a run is too long for them); with kDiag, the
+// accepted count. T threads per workgroup, U candidates per lane per trip.
+template <int T, int U, bool kLists, bool kMasks, bool kDiag>
 __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                const uint32_t *__restrict__ start, DensityFields f,
-                                               NeighbourList nl, SceneTable t, Consts k)
+                                               NeighbourList nl, uint64_t *__restrict__ masks, SceneTable t,
+                                               Consts k)
 {
     static_assert(kGroupSlots % T == 0, "a workgroup must lie in one slot group");
     __shared__ float lst[kListCap * T];
@@ -551,7 +644,7 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
     const uint32_t li = i - s.poff;
     if (i >= t.nslots || li >= s.n) return;
     float *my = lst + threadIdx.x;
-    uint32_t *nidx = nl.idx + i;
+    uint32_t *nidx = kLists ? nl.idx + list_slot(nl, i) : nullptr;
     const BufView bv = buf_view(p, f, s);
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     int x, y, z;
@@ -559,26 +652,32 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
     float density = 0.0f;
     uint32_t nc = 0, nn = 0;
     int cnt = 0;
-    walk_neighbours<U>(
-        s, k, start, bv, x, y, z, xi, yi, zi, cnt,
-        [&](uint32_t j, bool valid, float xj, float yj, float zj) {
-            const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
-            const float r2 = dx * dx + dyv * dyv + dzv * dzv;
-            const bool ok = valid && r2 <= k.hsq_f;
-            my[cnt * T] = r2;
-            cnt += ok;
-            nc += ok;
-            if (kLists && ok && j != li) {
-                if (nn < nl.kmax) nidx[(size_t) nn * nl.stride] = j;
-                ++nn;
-            }
-        },
-        [&]() {
-            for (int q = 0; q < cnt; ++q) {
-                const double wpoly = k.poly6 * cube_cr(k.hsq - (double) my[q * T]);
-                density = (float) ((double) density + wpoly);
-            }
-        });
+    MaskRecorder rec_masks;
+    rec_masks.masks = masks;
+    rec_masks.i = i;
+    auto test = [&](uint32_t j, bool valid, float xj, float yj, float zj) {
+        const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
+        const float r2 = dx * dx + dyv * dyv + dzv * dzv;
+        const bool ok = valid && r2 <= k.hsq_f;
+        my[cnt * T] = r2;
+        cnt += ok;
+        if (kDiag) nc += ok;
+        if (kLists && ok && j != li) {
+            if (nn < nl.kmax) nidx[(size_t) nn * 64] = j;
+            ++nn;
+        }
+        return ok;
+    };
+    auto flush = [&]() {
+        for (int q = 0; q < cnt; ++q) {
+            const double wpoly = k.poly6 * cube_cr(k.hsq - (double) my[q * T]);
+            density = (float) ((double) density + wpoly);
+        }
+    };
+    if constexpr (kMasks)
+        walk_neighbours<U>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush, rec_masks);
+    else
+        walk_neighbours<U>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush);
     density = density * k.mass;
     const float pressure = k.gas_k * (density - k.rho0);
     f.rho[i] = density;
@@ -594,7 +693,8 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
     rec[2] = make_float4(__uint_as_float((uint32_t) pb), __uint_as_float((uint32_t) (pb >> 32)),
                          __uint_as_float((uint32_t) rb), __uint_as_float((uint32_t) (rb >> 32)));
     if (kLists) nl.cnt[i] = nn;
-    if (f.ncount) f.ncount[i] = nc;
+    if (kMasks) nl.cnt[i] = rec_masks.overflow ? 1u : 0u;
+    if (kDiag) f.ncount[i] = nc;
 }
 
 struct ForceSums {
@@ -626,10 +726,17 @@ __device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
     return d;
 }
 
-// One accepted pair of updateForces, src/CCPUParticleSimulator.cpp:174-183, with
-// WspikyGradient / WviscosityLaplacian (src/CCPUParticleSimulator.cpp:17-30).
-__device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, const PairData &n, float xi, float yi,
-                                           float zi, float vxi, float vyi, float vzi, double ai, ForceSums &a)
+// The six float terms one accepted pair adds to the force sums
+// (src/CCPUParticleSimulator.cpp:174-183, with WspikyGradient /
+// WviscosityLaplacian, src/CCPUParticleSimulator.cpp:17-30): the pressure term
+// grad * pf and the viscosity term (v_j - v_i) * lap / rho_j, each rounded to
+// float before it is added, as the reference's QVector3D arithmetic does.
+struct PairTerms {
+    float px, py, pz, vx, vy, vz;
+};
+
+__device__ __forceinline__ PairTerms pair_terms(const BufView &bv, const Consts &k, const PairData &n, float xi,
+                                                float yi, float zi, float vxi, float vyi, float vzi, double ai)
 {
     const float dx = xi - n.x, dyv = yi - n.y, dzv = zi - n.z;
     const float r2 = dx * dx + dyv * dyv + dzv * dzv;
@@ -639,22 +746,135 @@ __device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, c
     float gx, gy, gz;
     div3_small_divisor(dx * sg, dyv * sg, dzv * sg, (float) r, gx, gy, gz);
     const float pf = (float) (ai + n.prr);
-    a.px = a.px + gx * pf;
-    a.py = a.py + gy * pf;
-    a.pz = a.pz + gz * pf;
+    PairTerms t;
+    t.px = gx * pf;
+    t.py = gy * pf;
+    t.pz = gz * pf;
     const float lap = (float) (k.visc * hr);
-    float tx, ty, tz;
     div3_rcp((n.vx - vxi) * lap, (n.vy - vyi) * lap, (n.vz - vzi) * lap, n.rinv,
-             [&]() { return ldf(bv.rho, n.j * 4u); }, tx, ty, tz);
-    a.vx = a.vx + tx;
-    a.vy = a.vy + ty;
-    a.vz = a.vz + tz;
+             [&]() { return ldf(bv.rho, n.j * 4u); }, t.vx, t.vy, t.vz);
+    return t;
 }
 
-// The force sums of particle i (slot li of its scene) followed by the
-// acceleration (f_p + f_v + f_g) / rho (src/CCPUParticleSimulator.cpp:155-193).
-// Neighbours come from the density pass's list when one was kept; otherwise (or
-// on overflow) the cells are walked again in the same order.
+__device__ __forceinline__ void add_terms(ForceSums &a, const PairTerms &t)
+{
+    a.px = a.px + t.px;
+    a.py = a.py + t.py;
+    a.pz = a.pz + t.pz;
+    a.vx = a.vx + t.vx;
+    a.vy = a.vy + t.vy;
+    a.vz = a.vz + t.vz;
+}
+
+// One accepted pair of updateForces, added to particle i's sums.
+__device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, const PairData &n, float xi, float yi,
+                                           float zi, float vxi, float vyi, float vzi, double ai, ForceSums &a)
+{
+    add_terms(a, pair_terms(bv, k, n, xi, yi, zi, vxi, vyi, vzi, ai));
+}
+
+// Drains a lane's LDS pair list `my` (stride T, cnt entries) into its force sums,
+// two pairs in flight so that the loads of one overlap the arithmetic of the other.
+template <int T>
+__device__ __forceinline__ void drain_pairs(const BufView &bv, const Consts &k, const uint32_t *my, int cnt, float xi,
+                                            float yi, float zi, float vxi, float vyi, float vzi, double ai,
+                                            ForceSums &a)
+{
+    if (cnt == 0) return;
+    int q = 0;
+    PairData pa = load_pair(bv, my[0]), pb;
+    for (; q + 1 < cnt; q += 2) {
+        pb = load_pair(bv, my[(q + 1) * T]);
+        force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
+        if (q + 2 < cnt) pa = load_pair(bv, my[(q + 2) * T]);
+        force_pair(bv, k, pb, xi, yi, zi, vxi, vyi, vzi, ai, a);
+    }
+    if (q < cnt) force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
+}
+
+// The force sums of particle i (slot li of its scene) from walking its cells,
+// in canonical order: each lane appends its accepted neighbours to its LDS list
+// `my` (stride T) and drains it through the pair body.
+template <int T, int U>
+__device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t *__restrict__ cell,
+                                               const uint32_t *__restrict__ start, uint32_t *my, uint32_t i,
+                                               uint32_t li, const Scene &s, const Consts &k, float xi, float yi,
+                                               float zi, float vxi, float vyi, float vzi, double ai)
+{
+    ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int x, y, z;
+    cell_xyz(cell[i], s, x, y, z);
+    int cnt = 0;
+    walk_neighbours<U>(
+        s, k, start, bv, x, y, z, xi, yi, zi, cnt,
+        [&](uint32_t j, bool valid, float xj, float yj, float zj) {
+            const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
+            const float r2 = dx * dx + dyv * dyv + dzv * dzv;
+            my[cnt * T] = j;
+            const bool ok = valid && r2 <= k.hsq_f && j != li;
+            cnt += ok;
+            return ok;
+        },
+        [&]() { drain_pairs<T>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a); });
+    return a;
+}
+
+// The force sums of particle i from the density pass's row masks: the same rows
+// and runs (row_run), and in each run only the accepted candidates, the set
+// bits in ascending order, so the pairs and their order are the walk's.
+template <int T>
+__device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t *__restrict__ cell,
+                                               const uint32_t *__restrict__ start,
+                                               const uint64_t *__restrict__ masks, uint32_t *my, uint32_t i,
+                                               uint32_t li, const Scene &s, const Consts &k, float xi, float yi,
+                                               float zi, float vxi, float vyi, float vzi, double ai)
+{
+    ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int x, y, z;
+    cell_xyz(cell[i], s, x, y, z);
+    const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
+    int cnt = 0;
+    for (int r = 0; r < 9; ++r) {
+        uint32_t j, jend;
+        row_run(rb, s, start, x, y, z, r, j, jend);
+        const uint32_t nw = (jend - j + 63u) >> 6;
+        for (uint32_t w = 0; __any(w < nw); ++w) {
+            uint64_t m = w < nw ? masks[mask_word(i, r, w)] : 0ull;
+            const uint32_t jw = j + 64u * w;
+            while (__any(m != 0ull)) {
+                if (m != 0ull) {
+                    const uint32_t jj = jw + (uint32_t) __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    my[cnt * T] = jj;
+                    cnt += jj != li;
+                }
+                if (__any(cnt > kListCap - 1)) {
+                    drain_pairs<T>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
+                    cnt = 0;
+                }
+            }
+        }
+    }
+    drain_pairs<T>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
+    return a;
+}
+
+// (f_p + f_v + f_g) / rho from the force sums (src/CCPUParticleSimulator.cpp:155-193).
+__device__ __forceinline__ void accel_from_sums(const ForceSums &a, float rhoi, const Scene &s, const Consts &k,
+                                                float &ax, float &ay, float &az)
+{
+    const float fac = k.neg_mass * rhoi;
+    const float fpx = a.px * fac, fpy = a.py * fac, fpz = a.pz * fac;
+    const float fvx = a.vx * k.mu_mass, fvy = a.vy * k.mu_mass, fvz = a.vz * k.mu_mass;
+    const float fgx = s.gx * rhoi, fgy = s.gy * rhoi, fgz = s.gz * rhoi;
+    ax = ((fpx + fvx) + fgx) / rhoi;
+    ay = ((fpy + fvy) + fgy) / rhoi;
+    az = ((fpz + fvz) + fgz) / rhoi;
+}
+
+// The acceleration of particle i. Neighbours come from the density pass's list
+// when one was kept (each lane running through its own list); otherwise (or on
+// overflow) the cells are walked again in the same order.
 template <int T, int U>
 __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *__restrict__ cell,
                                           const uint32_t *__restrict__ start, const DensityFields &f,
@@ -666,51 +886,106 @@ __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *
     const float rhoi = f.rho[i];
     const double ai = f.prr[i];
     ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    const uint32_t cnt_i = nl.kmax ? nl.cnt[i] : 0u;
-    if (nl.kmax && cnt_i <= nl.kmax) {
-        const uint32_t *nidx = nl.idx + i;
-        uint32_t j2 = cnt_i > 1 ? nidx[nl.stride] : 0u;
+    const uint32_t cnt_i = nl.mode == kNbLists ? nl.cnt[i] : 0u;
+    if (nl.mode == kNbLists && cnt_i <= nl.kmax) {
+        const uint32_t *nidx = nl.idx + list_slot(nl, i);
+        uint32_t j2 = cnt_i > 1 ? nidx[64] : 0u;
         PairData nxt = load_pair(bv, cnt_i ? nidx[0] : 0u);
         for (uint32_t q = 0; q < cnt_i; ++q) {
             const PairData cur = nxt;
             if (q + 1 < cnt_i) {
                 nxt = load_pair(bv, j2);
-                if (q + 2 < cnt_i) j2 = nidx[(size_t) (q + 2) * nl.stride];
+                if (q + 2 < cnt_i) j2 = nidx[(size_t) (q + 2) * 64];
             }
             force_pair(bv, k, cur, xi, yi, zi, vxi, vyi, vzi, ai, a);
         }
+    } else if (nl.mode == kNbMasks && nl.cnt[i] == 0) {
+        a = mask_sums<T>(bv, cell, start, nl.masks, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
     } else {
-        int x, y, z;
-        cell_xyz(cell[i], s, x, y, z);
-        int cnt = 0;
-        walk_neighbours<U>(
-            s, k, start, bv, x, y, z, xi, yi, zi, cnt,
-            [&](uint32_t j, bool valid, float xj, float yj, float zj) {
-                const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
-                const float r2 = dx * dx + dyv * dyv + dzv * dzv;
-                my[cnt * T] = j;
-                cnt += valid && r2 <= k.hsq_f && j != li;
-            },
-            [&]() {
-                // two pairs in flight: the loads of one overlap the arithmetic of the other
-                int q = 0;
-                PairData pa = load_pair(bv, my[0]), pb;
-                for (; q + 1 < cnt; q += 2) {
-                    pb = load_pair(bv, my[(q + 1) * T]);
-                    force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
-                    if (q + 2 < cnt) pa = load_pair(bv, my[(q + 2) * T]);
-                    force_pair(bv, k, pb, xi, yi, zi, vxi, vyi, vzi, ai, a);
-                }
-                if (q < cnt) force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
-            });
+        a = walk_sums<T, U>(bv, cell, start, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
     }
-    const float fac = k.neg_mass * rhoi;
-    const float fpx = a.px * fac, fpy = a.py * fac, fpz = a.pz * fac;
-    const float fvx = a.vx * k.mu_mass, fvy = a.vy * k.mu_mass, fvz = a.vz * k.mu_mass;
-    const float fgx = s.gx * rhoi, fgy = s.gy * rhoi, fgz = s.gz * rhoi;
-    ax = ((fpx + fvx) + fgx) / rhoi;
-    ay = ((fpy + fvy) + fgy) / rhoi;
-    az = ((fpz + fvz) + fgz) / rhoi;
+    accel_from_sums(a, rhoi, s, k, ax, ay, az);
+}
+
+// Force sums of a wave's 64 particles from their neighbour lists, with the pair
+// arithmetic spread over the whole wave. The pairs are taken level by level (the
+// k-th neighbour of every particle that has one, in lane order, then level k+1,
+// ...), 16 levels at a time: their (level, owner) items go to LDS, and each
+// round the active lanes compute the terms of the next items, one each,
+// whatever their owners. The terms are added to the owners' sums in LDS one level after the
+// other; an owner has at most one item per level, so no two lanes add to the
+// same sums at once, and every particle adds its terms in its own list order,
+// as the per-lane loop would. Lanes with ncnt = 0 take no part.
+constexpr uint32_t kLevelWindow = 16;
+
+__device__ __forceinline__ ForceSums wave_list_sums(const BufView &bv, const Consts &k, const NeighbourList &nl,
+                                                    uint32_t i, uint32_t ncnt, float xi, float yi, float zi,
+                                                    float vxi, float vyi, float vzi, double ai, float4 *s_owner,
+                                                    float4 *s_acc, uint16_t *s_items)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t ab = __builtin_bit_cast(uint64_t, ai);
+    s_owner[2 * lane] = make_float4(xi, yi, zi, vxi);
+    s_owner[2 * lane + 1] = make_float4(vyi, vzi, __uint_as_float((uint32_t) ab), __uint_as_float((uint32_t) (ab >> 32)));
+    s_acc[2 * lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s_acc[2 * lane + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t *wl = nl.idx + (size_t) (i >> 6) * nl.kmax * 64;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // lanes past the end of the slots have left: items are dealt to the active ones
+    const uint64_t active = __ballot(true);
+    const uint32_t nactive = (uint32_t) __popcll(active), rank = (uint32_t) __popcll(active & below);
+    for (uint32_t k0 = 0;; k0 += kLevelWindow) {
+        uint32_t nitems = 0;
+        for (uint32_t kk = k0; kk < k0 + kLevelWindow; ++kk) {
+            const uint64_t m = __ballot(ncnt > kk);
+            if (m == 0) break;
+            if (ncnt > kk) s_items[nitems + (uint32_t) __popcll(m & below)] = (uint16_t) (((kk - k0) << 6) | lane);
+            nitems += (uint32_t) __popcll(m);
+        }
+        if (nitems == 0) break;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t c = 0; c < nitems; c += nactive) {
+            const uint32_t q = c + rank;
+            const bool val = q < nitems;
+            const uint32_t item = val ? s_items[q] : 0u;
+            const uint32_t lev = item >> 6, o = item & 63u;
+            PairTerms tt = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (val) {
+                const float4 o0 = s_owner[2 * o], o1 = s_owner[2 * o + 1];
+                const double ao = __builtin_bit_cast(double, (uint64_t) __float_as_uint(o1.z) |
+                                                                 ((uint64_t) __float_as_uint(o1.w) << 32));
+                const PairData n = load_pair(bv, wl[(size_t) (k0 + lev) * 64 + o]);
+                tt = pair_terms(bv, k, n, o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, ao);
+            }
+            const uint32_t first_lev = __builtin_amdgcn_readfirstlane((uint32_t) s_items[c] >> 6);
+            const uint32_t last_q = c + nactive - 1 < nitems - 1 ? c + nactive - 1 : nitems - 1;
+            const uint32_t last_lev = __builtin_amdgcn_readfirstlane((uint32_t) s_items[last_q] >> 6);
+            for (uint32_t l = first_lev; l <= last_lev; ++l) {
+                if (val && lev == l) {
+                    float4 a0 = s_acc[2 * o], a1 = s_acc[2 * o + 1];
+                    a0.x = a0.x + tt.px;
+                    a0.y = a0.y + tt.py;
+                    a0.z = a0.z + tt.pz;
+                    a0.w = a0.w + tt.vx;
+                    a1.x = a1.x + tt.vy;
+                    a1.y = a1.y + tt.vz;
+                    s_acc[2 * o] = a0;
+                    s_acc[2 * o + 1] = a1;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    const float4 a0 = s_acc[2 * lane], a1 = s_acc[2 * lane + 1];
+    ForceSums a;
+    a.px = a0.x;
+    a.py = a0.y;
+    a.pz = a0.z;
+    a.vx = a0.w;
+    a.vy = a1.x;
+    a.vz = a1.y;
+    return a;
 }
 
 // integrate, src/CCPUParticleSimulator.cpp:220-224.
@@ -783,7 +1058,10 @@ __global__ __launch_bounds__(256) void k_integrate(ParticleSoA p, const float *_
 
 // Forces, walls and integration of one particle in one pass, plus the cell key of
 // its new position for the next step's sort (the dead cell for unused slots).
-template <int T, int U, bool kStoreAcc>
+// Neighbours: kNbWalk walks the cells; kNbLists takes the density pass's lists,
+// a wave at a time (wave_list_sums); kNbMasks iterates the density pass's row
+// masks (mask_sums). Particles whose list or masks overflowed walk their cells.
+template <int T, int U, int kMode, bool kStoreAcc>
 __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                     const uint32_t *__restrict__ start, DensityFields f,
                                                     NeighbourList nl, ParticleSoA out,
@@ -792,7 +1070,11 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
                                                     Consts k)
 {
     static_assert(kGroupSlots % T == 0, "a workgroup must lie in one slot group");
+    constexpr bool kLists = kMode == kNbLists;
     __shared__ uint32_t lst[kListCap * T];
+    __shared__ float4 s_owner[kLists ? 2 * T : 1];
+    __shared__ float4 s_acc[kLists ? 2 * T : 1];
+    __shared__ uint16_t s_items[kLists ? kLevelWindow * T : 1];
     const uint32_t first = xcd_block(blockIdx.x, gridDim.x) * (uint32_t) T, i = first + threadIdx.x;
     const Scene s = block_scene(t, first);
     const uint32_t li = i - s.poff;
@@ -804,8 +1086,31 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     const float vxi = p.vx[i], vyi = p.vy[i], vzi = p.vz[i];
     float a0, a1, a2;
-    sph_accel<T, U>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, a0,
-                            a1, a2);
+    if (kMode == kNbWalk) {
+        sph_accel<T, U>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1,
+                        a2);
+    } else {
+        const BufView bv = buf_view(p, f, s);
+        const float rhoi = f.rho[i];
+        const double ai = f.prr[i];
+        const uint32_t cnt_i = nl.cnt[i];
+        ForceSums a;
+        bool overflow;
+        if (kLists) {
+            overflow = cnt_i > nl.kmax;
+            const uint32_t w = threadIdx.x >> 6;
+            a = wave_list_sums(bv, k, nl, i, overflow ? 0u : cnt_i, xi, yi, zi, vxi, vyi, vzi, ai, s_owner + 128 * w,
+                               s_acc + 128 * w, s_items + kLevelWindow * 64 * w);
+        } else {
+            overflow = cnt_i != 0;
+            if (!overflow)
+                a = mask_sums<T>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi,
+                                 vzi, ai);
+        }
+        if (overflow)
+            a = walk_sums<T, U>(bv, cell, start, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
+        accel_from_sums(a, rhoi, s, k, a0, a1, a2);
+    }
     wall_accel(xi, yi, zi, vxi, vyi, vzi, s, k, a0, a1, a2);
     if (kStoreAcc) {
         ax[i] = a0;
@@ -824,26 +1129,51 @@ inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 // The compiled shapes of the two neighbour kernels, indexed by variant.
 template <int T, int U>
 struct NeighbourKernels {
-    static void density(hipStream_t st, bool lists, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
-                        const DensityFields &f, const NeighbourList &nl, const SceneTable &t, const Consts &k)
+    template <bool kLists, bool kMasks>
+    static void density_mode(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
+                             const DensityFields &f, const NeighbourList &nl, const SceneTable &t, const Consts &k)
     {
         const dim3 g(blocks_for(t.nslots, T));
-        if (lists)
-            hipLaunchKernelGGL((k_density<T, U, true>), g, dim3(T), 0, st, p, cell, start, f, nl, t, k);
+        if (f.ncount)
+            hipLaunchKernelGGL((k_density<T, U, kLists, kMasks, true>), g, dim3(T), 0, st, p, cell, start, f, nl,
+                               nl.masks, t, k);
         else
-            hipLaunchKernelGGL((k_density<T, U, false>), g, dim3(T), 0, st, p, cell, start, f, nl, t, k);
+            hipLaunchKernelGGL((k_density<T, U, kLists, kMasks, false>), g, dim3(T), 0, st, p, cell, start, f, nl,
+                               nl.masks, t, k);
+    }
+    static void density(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
+                        const DensityFields &f, const NeighbourList &nl, const SceneTable &t, const Consts &k)
+    {
+        if (nl.mode == kNbLists)
+            density_mode<true, false>(st, p, cell, start, f, nl, t, k);
+        else if (nl.mode == kNbMasks)
+            density_mode<false, true>(st, p, cell, start, f, nl, t, k);
+        else
+            density_mode<false, false>(st, p, cell, start, f, nl, t, k);
+    }
+    template <int kMode>
+    static void fused_mode(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
+                           const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out,
+                           uint32_t *key_next, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k)
+    {
+        const dim3 g(blocks_for(t.nslots, T));
+        if (ax)
+            hipLaunchKernelGGL((k_forces_fused<T, U, kMode, true>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
+                               key_next, ax, ay, az, t, k);
+        else
+            hipLaunchKernelGGL((k_forces_fused<T, U, kMode, false>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
+                               key_next, ax, ay, az, t, k);
     }
     static void fused(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
                       const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out, uint32_t *key_next,
                       float *ax, float *ay, float *az, const SceneTable &t, const Consts &k)
     {
-        const dim3 g(blocks_for(t.nslots, T));
-        if (ax)
-            hipLaunchKernelGGL((k_forces_fused<T, U, true>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
-                               key_next, ax, ay, az, t, k);
+        if (nl.mode == kNbLists)
+            fused_mode<kNbLists>(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k);
+        else if (nl.mode == kNbMasks)
+            fused_mode<kNbMasks>(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k);
         else
-            hipLaunchKernelGGL((k_forces_fused<T, U, false>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
-                               key_next, ax, ay, az, t, k);
+            fused_mode<kNbWalk>(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k);
     }
 };
 
@@ -922,13 +1252,12 @@ void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *
                     const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, hipStream_t st)
 {
     if (t.nslots == 0) return;
-    const bool lists = nl.kmax != 0;
     switch (variant) {
 #define SPH_DENSITY_CASE(i, t_, u) \
-    case i: NeighbourKernels<t_, u>::density(st, lists, p, cell, start, f, nl, t, k); break;
+    case i: NeighbourKernels<t_, u>::density(st, p, cell, start, f, nl, t, k); break;
         SPH_VARIANTS(SPH_DENSITY_CASE)
 #undef SPH_DENSITY_CASE
-    default: NeighbourKernels<64, 4>::density(st, lists, p, cell, start, f, nl, t, k); break;
+    default: NeighbourKernels<64, 4>::density(st, p, cell, start, f, nl, t, k); break;
     }
 }
 

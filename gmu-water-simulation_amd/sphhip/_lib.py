@@ -22,7 +22,8 @@ NEIGHBOUR_LISTS_OFF = 0xFFFFFFFF
 STAGES = ["hash", "sort", "cells", "reorder", "density", "forces", "collisions", "integrate"]
 EMIT_FOUNTAIN, EMIT_DISC = 0, 1
 NUM_VARIANTS = 8
-DEFAULT_VARIANT = 2
+DEFAULT_VARIANT = 7
+NEIGHBOURS_WALK, NEIGHBOURS_LISTS, NEIGHBOURS_MASKS = 0, 1, 2
 GROUP_SLOTS = 256
 
 # Every entry point of include/sph_hip.h with its ctypes signature.

@@ -176,22 +176,23 @@ class SceneBatch:
     def set_graphs(self, enable):
         _lib.check(self._lib.sph_set_graphs(self._ctx, int(bool(enable))))
 
-    def set_variant(self, variant, lists=False):
-        _lib.check(self._lib.sph_set_variant(self._ctx, int(variant), int(bool(lists))))
+    def set_variant(self, variant, mode=_lib.NEIGHBOURS_WALK):
+        """Kernel shape `variant` and neighbour mode (NEIGHBOURS_WALK, _LISTS, _MASKS)."""
+        _lib.check(self._lib.sph_set_variant(self._ctx, int(variant), int(mode)))
 
     def variant(self):
-        v, l = C.c_int(0), C.c_int(0)
-        _lib.check(self._lib.sph_get_variant(self._ctx, C.byref(v), C.byref(l)))
-        return v.value, bool(l.value)
+        v, m = C.c_int(0), C.c_int(0)
+        _lib.check(self._lib.sph_get_variant(self._ctx, C.byref(v), C.byref(m)))
+        return v.value, m.value
 
     def autotune(self, steps=5):
-        """Time every (variant, lists) choice for `steps` steps from the current
-        state, keep the fastest and leave the state unchanged. Returns
-        {(variant, lists): ms_per_step} for the choices that ran."""
-        ms = np.zeros(2 * _lib.NUM_VARIANTS, np.float32)
+        """Time every (variant, neighbour mode) choice for `steps` steps from the
+        current state, keep the fastest and leave the state unchanged. Returns
+        {(variant, mode): ms_per_step} for the choices that ran."""
+        ms = np.zeros(3 * _lib.NUM_VARIANTS, np.float32)
         _lib.check(self._lib.sph_autotune(self._ctx, int(steps), _lib.ptr(ms)))
-        return {(v, bool(l)): float(ms[2 * v + l]) for v in range(_lib.NUM_VARIANTS) for l in (0, 1)
-                if ms[2 * v + l] >= 0}
+        return {(v, m): float(ms[3 * v + m]) for v in range(_lib.NUM_VARIANTS) for m in (0, 1, 2)
+                if ms[3 * v + m] >= 0}
 
 
 def variant_shape(variant):

@@ -85,10 +85,11 @@ typedef struct {
     float dt;           /* 0.01f in the reference (src/CBaseParticleSimulator.cpp:7) */
     uint32_t capacity;  /* maximum particle count (m_maxParticlesCount) */
     uint32_t flags;     /* SPH_FLAG_* */
-    uint32_t max_neighbours; /* 0 (default) or SPH_NEIGHBOUR_LISTS_OFF: the force pass walks the cells
-                                itself. N > 0: the density pass keeps lists of up to N neighbours per
-                                particle for the force pass (N*4 bytes per particle of HBM); particles
-                                with more neighbours are still exact, only slower. */
+    uint32_t max_neighbours; /* 0 (default) or SPH_NEIGHBOUR_LISTS_OFF: no lists (the force pass reads
+                                the density pass's row masks, see sph_set_variant). N > 0: the density
+                                pass keeps lists of up to N neighbours per particle for the force pass
+                                (N*4 bytes per particle of HBM); particles with more neighbours are
+                                still exact, only slower. */
 } sph_params;
 
 #define SPH_NEIGHBOUR_LISTS_OFF 0xFFFFFFFFu
@@ -277,19 +278,26 @@ sph_status sph_scene_set_emitter(sph_ctx *ctx, uint32_t scene, const sph_emitter
 sph_status sph_set_graphs(sph_ctx *ctx, int enable);
 
 /* Shape of the neighbour kernels: variant 0..SPH_NUM_VARIANTS-1 (workgroup size and
- * candidates per trip, see sph_variant_shape; a new context uses variant 2) and lists != 0 to keep neighbour lists
- * (only if the context was created with max_neighbours > 0). Results are identical for
- * every choice. */
+ * candidates per trip, see sph_variant_shape), and how the
+ * force pass finds the neighbours:
+ *   SPH_NEIGHBOURS_WALK  tests every candidate of the 27 cells again;
+ *   SPH_NEIGHBOURS_LISTS reads neighbour lists the density pass kept (only if the context
+ *                        was created with max_neighbours > 0);
+ *   SPH_NEIGHBOURS_MASKS reads per-row masks of the accepted candidates the density pass
+ *                        kept (288 bytes per slot).
+ * A new context uses variant 7 and SPH_NEIGHBOURS_MASKS, or SPH_NEIGHBOURS_LISTS when it
+ * was created with max_neighbours > 0. Results are identical for every choice. */
 #define SPH_NUM_VARIANTS 8
-sph_status sph_set_variant(sph_ctx *ctx, int variant, int lists);
-sph_status sph_get_variant(const sph_ctx *ctx, int *variant, int *lists);
+enum { SPH_NEIGHBOURS_WALK = 0, SPH_NEIGHBOURS_LISTS = 1, SPH_NEIGHBOURS_MASKS = 2 };
+sph_status sph_set_variant(sph_ctx *ctx, int variant, int mode);
+sph_status sph_get_variant(const sph_ctx *ctx, int *variant, int *mode);
 /* Threads per workgroup and candidates tested per lane per trip of a variant. */
 sph_status sph_variant_shape(int variant, int *threads, int *unroll);
 
-/* Times `steps` steps of every (variant, lists) choice from the current state, keeps the
- * fastest and restores the state, so the call changes nothing but the choice. ms_out
- * (may be NULL) receives 2*SPH_NUM_VARIANTS ms-per-step figures, index 2*variant + lists,
- * negative where a choice was not run. */
+/* Times `steps` steps of every (variant, neighbour mode) choice from the current state,
+ * keeps the fastest and restores the state, so the call changes nothing but the choice.
+ * ms_out (may be NULL) receives 3*SPH_NUM_VARIANTS ms-per-step figures, index
+ * 3*variant + mode, negative where a choice was not run. */
 sph_status sph_autotune(sph_ctx *ctx, uint32_t steps, float *ms_out);
 
 /* ------------------------------------------------------------------------------

@@ -5,8 +5,9 @@ Bars:
   * every scene of a batch is bitwise what it is alone in a context of its own
     (same positions, velocities, cells, permutation and cell table), and alone
     contexts are bitwise the canonical-order oracle (test_gpu_parity.py);
-  * graph replays, every kernel variant, neighbour lists on or off, and a state
-    that went through the autotuner give bitwise the same trajectories;
+  * graph replays, every kernel variant, every neighbour mode of the force pass
+    (walk, lists, row masks), and a state that went through the autotuner give
+    bitwise the same trajectories;
   * the device fountain is bitwise the oracle's fountain (the reference's
     generateParticles, src/CBaseParticleSimulator.cpp:187-210), and the device
     disc emitter is bitwise a host append of the oracle's restatement of it.
@@ -130,22 +131,40 @@ def test_single_scene_graphs_equal_eager(hip):
     b.close()
 
 
-def test_variants_and_lists_identical(hip):
+def test_variants_and_modes_identical(hip):
     scs = small_scenes(hip, count=3, seed=8)
     ref = None
-    for lists in (False, True):
+    for mode in (hip._lib.NEIGHBOURS_WALK, hip._lib.NEIGHBOURS_LISTS, hip._lib.NEIGHBOURS_MASKS):
         for v in range(hip._lib.NUM_VARIANTS):
             with make_batch(hip, scs, max_neighbours=64) as b:
-                b.set_variant(v, lists)
-                assert b.variant() == (v, lists)
+                b.set_variant(v, mode)
+                assert b.variant() == (v, mode)
                 b.step(8)
                 got = [b.download(i)[:2] for i in range(len(scs))]
             if ref is None:
                 ref = got
                 continue
             for i in range(len(scs)):
-                assert_bits_equal(got[i][0], ref[i][0], "variant %d lists %d scene %d pos" % (v, lists, i))
-                assert_bits_equal(got[i][1], ref[i][1], "variant %d lists %d scene %d vel" % (v, lists, i))
+                assert_bits_equal(got[i][0], ref[i][0], "variant %d mode %d scene %d pos" % (v, mode, i))
+                assert_bits_equal(got[i][1], ref[i][1], "variant %d mode %d scene %d vel" % (v, mode, i))
+
+
+def test_masks_with_overflowing_runs(hip, orc):
+    """Row masks hold 256 candidates per row; a crowded scene whose runs are longer
+    walks those particles instead: still bitwise the oracle."""
+    rng = np.random.default_rng(5)
+    pos = (rng.random((3000, 3), dtype=np.float32) * np.float32(0.09) - np.float32(0.2)).astype(np.float32)
+    with hip.SceneBatch([(0.4, len(pos))]) as b:
+        b.upload(0, pos)
+        b.set_variant(2, hip._lib.NEIGHBOURS_MASKS)
+        b.step(3)
+        p, v, _ = b.download(0)
+    o = orc.OracleSim(0.4, len(pos))
+    o.add_particles(pos)
+    o.set_order_mode(orc.ORDER_CANONICAL)
+    o.step(3)
+    assert_bits_equal(p, o.state()["pos"], "crowded scene with masks")
+    assert_bits_equal(v, o.state()["vel"], "crowded scene with masks (velocities)")
 
 
 def test_autotune_changes_only_the_choice(hip):
@@ -155,9 +174,9 @@ def test_autotune_changes_only_the_choice(hip):
         b.step(3)
         times = a.autotune(steps=2)
         print("autotune ms/step:", {k: round(t, 4) for k, t in times.items()}, flush=True)
-        assert len(times) == 2 * hip._lib.NUM_VARIANTS
-        v, lists = a.variant()
-        assert times[(v, lists)] == min(times.values())
+        assert len(times) == 3 * hip._lib.NUM_VARIANTS
+        v, mode = a.variant()
+        assert times[(v, mode)] == min(times.values())
         a.step(10)
         b.step(10)
         for i in range(len(scs)):
@@ -285,7 +304,7 @@ def test_batch_errors(hip):
         with pytest.raises(hip.SphError):
             b.set_variant(hip._lib.NUM_VARIANTS)
         with pytest.raises(hip.SphError):
-            b.set_variant(0, lists=True)    # created without lists
+            b.set_variant(0, hip._lib.NEIGHBOURS_LISTS)    # created without lists
         with pytest.raises(hip.SphError):
             b.download(1)                   # no scene 1
         with pytest.raises(hip.SphError):
