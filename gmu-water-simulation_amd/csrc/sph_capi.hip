@@ -1381,12 +1381,14 @@ sph_status sph_set_graphs(sph_ctx *c, int enable)
     return SPH_OK;
 }
 
-sph_status sph_variant_shape(int variant, int *threads, int *unroll)
+sph_status sph_variant_shape(int variant, int *threads, int *unroll, int *pairs, int *stall)
 {
     if (variant < 0 || variant >= sph::kNumVariants)
         return fail(SPH_ERR_INVALID, "sph_variant_shape: no variant " + std::to_string(variant));
     if (threads) *threads = sph::kNeighbourVariants[variant].threads;
     if (unroll) *unroll = sph::kNeighbourVariants[variant].unroll;
+    if (pairs) *pairs = sph::kNeighbourVariants[variant].pairs;
+    if (stall) *stall = sph::kNeighbourVariants[variant].stall;
     return SPH_OK;
 }
 

@@ -62,14 +62,19 @@ constexpr int kScanTile = 4096;  // elements per 1024-thread scan block
 inline uint32_t sort_blocks(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
 inline uint32_t scan_blocks(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 
-// The tunable shape of the two neighbour kernels: workgroup size and candidates
-// tested per lane per trip. Results do not depend on the variant.
+// The tunable shape of the two neighbour kernels: workgroup size, candidates
+// tested per lane per trip, and the force pass's per-lane pair list in the mask
+// mode: its capacity and drain policy (0: drain when any list is full; P: full
+// lanes wait until P percent of the wave waits). Results do not depend on the
+// variant.
 struct NeighbourVariant {
     int threads;
     int unroll;
+    int pairs;
+    int stall;
 };
-constexpr int kNumVariants = 8;
-constexpr int kDefaultVariant = 7;   // 64 threads, 8 candidates per trip: fastest measured with row masks
+constexpr int kNumVariants = 12;
+constexpr int kDefaultVariant = 8;   // 64 threads, 8 candidates, 32 pairs, 75 %: fastest measured with row masks
 extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 
 // Emitters (one workgroup each) add particles at the start of a step and update

@@ -821,8 +821,13 @@ __device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t
 
 // The force sums of particle i from the density pass's row masks: the same rows
 // and runs (row_run), and in each run only the accepted candidates, the set
-// bits in ascending order, so the pairs and their order are the walk's.
-template <int T>
+// bits in ascending order, so the pairs and their order are the walk's. The
+// lane's pair list `my` holds C entries (stride T); a longer list means fewer,
+// fuller drains. kStall == 0 drains all lists when one is full; kStall == P
+// lets full lanes wait (keeping their bits) until P percent of the wave waits,
+// or until every lane with bits left waits, so that drains run with most lanes
+// busy. Each lane's pairs keep their order either way.
+template <int T, int C, int kStall>
 __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t *__restrict__ cell,
                                                const uint32_t *__restrict__ start,
                                                const uint64_t *__restrict__ masks, uint32_t *my, uint32_t i,
@@ -842,13 +847,23 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
             uint64_t m = w < nw ? masks[mask_word(i, r, w)] : 0ull;
             const uint32_t jw = j + 64u * w;
             while (__any(m != 0ull)) {
-                if (m != 0ull) {
+                // kStall: a lane whose list is full keeps its bits until the drain
+                if (m != 0ull && (kStall == 0 || cnt < C)) {
                     const uint32_t jj = jw + (uint32_t) __builtin_ctzll(m);
                     m &= m - 1ull;
                     my[cnt * T] = jj;
                     cnt += jj != li;
                 }
-                if (__any(cnt > kListCap - 1)) {
+                bool drain;
+                if (kStall == 0) {
+                    drain = __any(cnt > C - 1);
+                } else {
+                    const bool full = cnt == C;
+                    const uint32_t waiting = (uint32_t) __popcll(__ballot(full));
+                    const uint32_t lanes = (uint32_t) __popcll(__ballot(true));
+                    drain = waiting * 100u >= (uint32_t) kStall * lanes || (waiting && !__any(m != 0ull && !full));
+                }
+                if (drain) {
                     drain_pairs<T>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
                     cnt = 0;
                 }
@@ -900,7 +915,7 @@ __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *
             force_pair(bv, k, cur, xi, yi, zi, vxi, vyi, vzi, ai, a);
         }
     } else if (nl.mode == kNbMasks && nl.cnt[i] == 0) {
-        a = mask_sums<T>(bv, cell, start, nl.masks, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
+        a = mask_sums<T, kListCap, 0>(bv, cell, start, nl.masks, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
     } else {
         a = walk_sums<T, U>(bv, cell, start, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
     }
@@ -1061,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_integrate(ParticleSoA p, const float *_
 // Neighbours: kNbWalk walks the cells; kNbLists takes the density pass's lists,
 // a wave at a time (wave_list_sums); kNbMasks iterates the density pass's row
 // masks (mask_sums). Particles whose list or masks overflowed walk their cells.
-template <int T, int U, int kMode, bool kStoreAcc>
+template <int T, int U, int C, int kStall, int kMode, bool kStoreAcc>
 __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                     const uint32_t *__restrict__ start, DensityFields f,
                                                     NeighbourList nl, ParticleSoA out,
@@ -1071,7 +1086,7 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
 {
     static_assert(kGroupSlots % T == 0, "a workgroup must lie in one slot group");
     constexpr bool kLists = kMode == kNbLists;
-    __shared__ uint32_t lst[kListCap * T];
+    __shared__ uint32_t lst[(kMode == kNbMasks && C > kListCap ? C : kListCap) * T];
     __shared__ float4 s_owner[kLists ? 2 * T : 1];
     __shared__ float4 s_acc[kLists ? 2 * T : 1];
     __shared__ uint16_t s_items[kLists ? kLevelWindow * T : 1];
@@ -1104,8 +1119,8 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
         } else {
             overflow = cnt_i != 0;
             if (!overflow)
-                a = mask_sums<T>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi,
-                                 vzi, ai);
+                a = mask_sums<T, C, kStall>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
+                                            vxi, vyi, vzi, ai);
         }
         if (overflow)
             a = walk_sums<T, U>(bv, cell, start, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
@@ -1127,7 +1142,7 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
 inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 
 // The compiled shapes of the two neighbour kernels, indexed by variant.
-template <int T, int U>
+template <int T, int U, int C, int kStall>
 struct NeighbourKernels {
     template <bool kLists, bool kMasks>
     static void density_mode(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
@@ -1158,11 +1173,11 @@ struct NeighbourKernels {
     {
         const dim3 g(blocks_for(t.nslots, T));
         if (ax)
-            hipLaunchKernelGGL((k_forces_fused<T, U, kMode, true>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
-                               key_next, ax, ay, az, t, k);
+            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kMode, true>), g, dim3(T), 0, st, p, cell, start, f,
+                               nl, out, key_next, ax, ay, az, t, k);
         else
-            hipLaunchKernelGGL((k_forces_fused<T, U, kMode, false>), g, dim3(T), 0, st, p, cell, start, f, nl, out,
-                               key_next, ax, ay, az, t, k);
+            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kMode, false>), g, dim3(T), 0, st, p, cell, start, f,
+                               nl, out, key_next, ax, ay, az, t, k);
     }
     static void fused(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
                       const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out, uint32_t *key_next,
@@ -1177,20 +1192,24 @@ struct NeighbourKernels {
     }
 };
 
-// X(index, threads, unroll)
-#define SPH_VARIANTS(X) \
-    X(0, 256, 4)        \
-    X(1, 128, 4)        \
-    X(2, 64, 4)         \
-    X(3, 256, 2)        \
-    X(4, 128, 2)        \
-    X(5, 256, 8)        \
-    X(6, 64, 2)         \
-    X(7, 64, 8)
+// X(index, threads, unroll, pair-list capacity and drain policy of the mask mode)
+#define SPH_VARIANTS(X)      \
+    X(0, 256, 4, 16, 0)      \
+    X(1, 128, 4, 16, 0)      \
+    X(2, 64, 4, 16, 0)       \
+    X(3, 256, 2, 16, 0)      \
+    X(4, 64, 8, 16, 0)       \
+    X(5, 64, 4, 32, 0)       \
+    X(6, 64, 8, 32, 0)       \
+    X(7, 64, 8, 32, 50)      \
+    X(8, 64, 8, 32, 75)      \
+    X(9, 64, 4, 32, 75)      \
+    X(10, 64, 8, 24, 75)     \
+    X(11, 64, 8, 64, 0)
 
 }  // namespace
 
-#define SPH_VARIANT_ROW(i, t, u) {t, u},
+#define SPH_VARIANT_ROW(i, t, u, c, st) {t, u, c, st},
 const NeighbourVariant kNeighbourVariants[kNumVariants] = {SPH_VARIANTS(SPH_VARIANT_ROW)};
 #undef SPH_VARIANT_ROW
 
@@ -1253,11 +1272,11 @@ void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *
 {
     if (t.nslots == 0) return;
     switch (variant) {
-#define SPH_DENSITY_CASE(i, t_, u) \
-    case i: NeighbourKernels<t_, u>::density(st, p, cell, start, f, nl, t, k); break;
+#define SPH_DENSITY_CASE(i, t_, u, c, stl) \
+    case i: NeighbourKernels<t_, u, c, stl>::density(st, p, cell, start, f, nl, t, k); break;
         SPH_VARIANTS(SPH_DENSITY_CASE)
 #undef SPH_DENSITY_CASE
-    default: NeighbourKernels<64, 4>::density(st, p, cell, start, f, nl, t, k); break;
+    default: NeighbourKernels<64, 8, 32, 0>::density(st, p, cell, start, f, nl, t, k); break;
     }
 }
 
@@ -1291,11 +1310,11 @@ void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint3
 {
     if (t.nslots == 0) return;
     switch (variant) {
-#define SPH_FUSED_CASE(i, t_, u) \
-    case i: NeighbourKernels<t_, u>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+#define SPH_FUSED_CASE(i, t_, u, c, stl) \
+    case i: NeighbourKernels<t_, u, c, stl>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
         SPH_VARIANTS(SPH_FUSED_CASE)
 #undef SPH_FUSED_CASE
-    default: NeighbourKernels<64, 4>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    default: NeighbourKernels<64, 8, 32, 0>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
     }
 }
 

@@ -196,7 +196,8 @@ class SceneBatch:
 
 
 def variant_shape(variant):
-    """(threads per workgroup, candidates per lane per trip)."""
-    t, u = C.c_int(0), C.c_int(0)
-    _lib.check(_lib.load().sph_variant_shape(int(variant), C.byref(t), C.byref(u)))
-    return t.value, u.value
+    """(threads per workgroup, candidates per lane per trip, mask-mode pair list
+    capacity, mask-mode drain stall percent)."""
+    t, u, c, st = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    _lib.check(_lib.load().sph_variant_shape(int(variant), C.byref(t), C.byref(u), C.byref(c), C.byref(st)))
+    return t.value, u.value, c.value, st.value
