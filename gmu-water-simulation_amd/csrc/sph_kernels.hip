@@ -773,23 +773,25 @@ __device__ __forceinline__ void force_pair(const BufView &bv, const Consts &k, c
     add_terms(a, pair_terms(bv, k, n, xi, yi, zi, vxi, vyi, vzi, ai));
 }
 
-// Drains a lane's LDS pair list `my` (stride T, cnt entries) into its force sums,
-// two pairs in flight so that the loads of one overlap the arithmetic of the other.
-template <int T>
+// Drains a lane's LDS pair list `my` (stride T, cnt <= C entries) into its force
+// sums, in list order. Two pair buffers alternate, each reloaded as soon as it
+// has been used, so the loads of one pair overlap the arithmetic of the other.
+// The loads are unconditional (indices past the end are clamped into the list
+// and their pairs not added), which keeps the buffers in place instead of
+// merging masked loads into copies.
+template <int T, int C>
 __device__ __forceinline__ void drain_pairs(const BufView &bv, const Consts &k, const uint32_t *my, int cnt, float xi,
                                             float yi, float zi, float vxi, float vyi, float vzi, double ai,
                                             ForceSums &a)
 {
     if (cnt == 0) return;
-    int q = 0;
-    PairData pa = load_pair(bv, my[0]), pb;
-    for (; q + 1 < cnt; q += 2) {
-        pb = load_pair(bv, my[(q + 1) * T]);
+    PairData pa = load_pair(bv, my[0]);
+    for (int q = 0; q < cnt; q += 2) {
+        const PairData pb = load_pair(bv, my[(q + 1 < C ? q + 1 : C - 1) * T]);
         force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
-        if (q + 2 < cnt) pa = load_pair(bv, my[(q + 2) * T]);
-        force_pair(bv, k, pb, xi, yi, zi, vxi, vyi, vzi, ai, a);
+        pa = load_pair(bv, my[(q + 2 < C ? q + 2 : C - 1) * T]);
+        if (q + 1 < cnt) force_pair(bv, k, pb, xi, yi, zi, vxi, vyi, vzi, ai, a);
     }
-    if (q < cnt) force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
 }
 
 // The force sums of particle i (slot li of its scene) from walking its cells,
@@ -815,7 +817,7 @@ __device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t
             cnt += ok;
             return ok;
         },
-        [&]() { drain_pairs<T>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a); });
+        [&]() { drain_pairs<T, kListCap>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a); });
     return a;
 }
 
@@ -864,13 +866,13 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
                     drain = waiting * 100u >= (uint32_t) kStall * lanes || (waiting && !__any(m != 0ull && !full));
                 }
                 if (drain) {
-                    drain_pairs<T>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
+                    drain_pairs<T, C>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
                     cnt = 0;
                 }
             }
         }
     }
-    drain_pairs<T>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
+    drain_pairs<T, C>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
     return a;
 }
 
