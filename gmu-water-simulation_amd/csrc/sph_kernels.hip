@@ -840,39 +840,58 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
     int x, y, z;
     cell_xyz(cell[i], s, x, y, z);
     const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
+    // rows and words advance together for the wave (r, w uniform); m holds the
+    // lane's unread bits of word w of row r, whose first candidate is jw
+    int r = -1;
+    uint32_t w = 0, nw = 0, jrun = 0, jw = 0;
+    uint64_t m = 0;
+    bool done = false;
     int cnt = 0;
-    for (int r = 0; r < 9; ++r) {
-        uint32_t j, jend;
-        row_run(rb, s, start, x, y, z, r, j, jend);
-        const uint32_t nw = (jend - j + 63u) >> 6;
-        for (uint32_t w = 0; __any(w < nw); ++w) {
-            uint64_t m = w < nw ? masks[mask_word(i, r, w)] : 0ull;
-            const uint32_t jw = j + 64u * w;
-            while (__any(m != 0ull)) {
-                // kStall: a lane whose list is full keeps its bits until the drain
+    for (;;) {
+        // fill the pair lists until a drain is due; the sums are not touched here
+        for (;;) {
+            if (!__any(m != 0ull)) {
+                ++w;
+                if (!__any(w < nw)) {
+                    if (++r == 9) {
+                        done = true;
+                        break;
+                    }
+                    uint32_t jend;
+                    row_run(rb, s, start, x, y, z, r, jrun, jend);
+                    nw = (jend - jrun + 63u) >> 6;
+                    w = 0;
+                }
+                m = w < nw ? masks[mask_word(i, r, w)] : 0ull;
+                jw = jrun + 64u * w;
+                continue;
+            }
+            // two bits per round; kStall: a lane whose list is full keeps its
+            // bits until the drain (without kStall a list has room for both)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
                 if (m != 0ull && (kStall == 0 || cnt < C)) {
                     const uint32_t jj = jw + (uint32_t) __builtin_ctzll(m);
                     m &= m - 1ull;
                     my[cnt * T] = jj;
                     cnt += jj != li;
                 }
-                bool drain;
-                if (kStall == 0) {
-                    drain = __any(cnt > C - 1);
-                } else {
-                    const bool full = cnt == C;
-                    const uint32_t waiting = (uint32_t) __popcll(__ballot(full));
-                    const uint32_t lanes = (uint32_t) __popcll(__ballot(true));
-                    drain = waiting * 100u >= (uint32_t) kStall * lanes || (waiting && !__any(m != 0ull && !full));
-                }
-                if (drain) {
-                    drain_pairs<T, C>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
-                    cnt = 0;
-                }
             }
+            bool drain;
+            if (kStall == 0) {
+                drain = __any(cnt > C - 2);
+            } else {
+                const bool full = cnt == C;
+                const uint32_t waiting = (uint32_t) __popcll(__ballot(full));
+                const uint32_t lanes = (uint32_t) __popcll(__ballot(true));
+                drain = waiting * 100u >= (uint32_t) kStall * lanes || (waiting && !__any(m != 0ull && !full));
+            }
+            if (drain) break;
         }
+        drain_pairs<T, C>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
+        cnt = 0;
+        if (done) break;
     }
-    drain_pairs<T, C>(bv, k, my, cnt, xi, yi, zi, vxi, vyi, vzi, ai, a);
     return a;
 }
 
