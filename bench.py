@@ -128,7 +128,7 @@ def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
                        "boxes at 1/4 fill with per-axis aspect U[0.5, 2]" % (nscenes, seed),
            "scenes": nscenes, "particles": total, "steps": steps, "value": total * steps / wall,
            "unit": "particle-steps/s", "ms_per_step": wall * 1e3 / steps, "graphs": True,
-           "variant": {"index": variant, "threads_unroll_pairs_stall": sphhip.variant_shape(variant),
+           "variant": {"index": variant, "threads_unroll_pairs_stall_queue": sphhip.variant_shape(variant),
                        "neighbours": ("walk", "lists", "masks")[mode]},
            "autotune_ms_per_step": {"%d%s" % (v, "wlm"[m]): round(ms, 4) for (v, m), ms in tune.items()}}
     if per_scene_steps:

@@ -1381,7 +1381,7 @@ sph_status sph_set_graphs(sph_ctx *c, int enable)
     return SPH_OK;
 }
 
-sph_status sph_variant_shape(int variant, int *threads, int *unroll, int *pairs, int *stall)
+sph_status sph_variant_shape(int variant, int *threads, int *unroll, int *pairs, int *stall, int *queue)
 {
     if (variant < 0 || variant >= sph::kNumVariants)
         return fail(SPH_ERR_INVALID, "sph_variant_shape: no variant " + std::to_string(variant));
@@ -1389,6 +1389,7 @@ sph_status sph_variant_shape(int variant, int *threads, int *unroll, int *pairs,
     if (unroll) *unroll = sph::kNeighbourVariants[variant].unroll;
     if (pairs) *pairs = sph::kNeighbourVariants[variant].pairs;
     if (stall) *stall = sph::kNeighbourVariants[variant].stall;
+    if (queue) *queue = sph::kNeighbourVariants[variant].queue;
     return SPH_OK;
 }
 

@@ -63,18 +63,20 @@ inline uint32_t sort_blocks(uint32_t n) { return (n + kSortTile - 1) / kSortTile
 inline uint32_t scan_blocks(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 
 // The tunable shape of the two neighbour kernels: workgroup size, candidates
-// tested per lane per trip, and the force pass's per-lane pair list in the mask
-// mode: its capacity and drain policy (0: drain when any list is full; P: full
-// lanes wait until P percent of the wave waits). Results do not depend on the
-// variant.
+// tested per lane per trip, and how the force pass reads the row masks: with
+// `pairs` 32-bit LDS words per lane either as a pair list with a drain policy
+// (stall 0: drain when any list is full; P: full lanes wait until P percent of
+// the wave waits) or, with `queue`, as a queue of pairs / 3 mask words.
+// Results do not depend on the variant.
 struct NeighbourVariant {
     int threads;
     int unroll;
     int pairs;
     int stall;
+    int queue;
 };
-constexpr int kNumVariants = 12;
-constexpr int kDefaultVariant = 8;   // 64 threads, 8 candidates, 32 pairs, 75 %: fastest measured with row masks
+constexpr int kNumVariants = 14;
+constexpr int kDefaultVariant = 12;  // 64 threads, 8 candidates, a 16-word mask queue: fastest measured on the 1M dam break
 extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 
 // Emitters (one workgroup each) add particles at the start of a step and update

@@ -895,6 +895,90 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
     return a;
 }
 
+// Drains a lane's queue of mask words (nq entries of three 32-bit words at
+// stride T: bits 0-31, bits 32-63, slot of bit 0) into its force sums: the set
+// bits of each word in ascending order, word after word. Two pairs are in
+// flight, as in drain_pairs; a lane with no pair left loads `fallback` and adds
+// nothing.
+template <int T>
+__device__ __forceinline__ void drain_words(const BufView &bv, const Consts &k, const uint32_t *qw, int nq,
+                                            uint32_t fallback, float xi, float yi, float zi, float vxi, float vyi,
+                                            float vzi, double ai, ForceSums &a)
+{
+    int q = 0;
+    uint64_t m = 0;
+    uint32_t j0 = 0;
+    auto next = [&](uint32_t &j) {
+        if (m == 0ull && q < nq) {
+            m = (uint64_t) qw[(3 * q) * T] | ((uint64_t) qw[(3 * q + 1) * T] << 32);
+            j0 = qw[(3 * q + 2) * T];
+            ++q;
+        }
+        const bool ok = m != 0ull;
+        j = ok ? j0 + (uint32_t) __builtin_ctzll(m) : fallback;
+        m &= m - 1ull;
+        return ok;
+    };
+    uint32_t ja, jb;
+    bool va = next(ja);
+    PairData pa = load_pair(bv, ja);
+    while (__any(va)) {
+        const bool vb = next(jb);
+        const PairData pb = load_pair(bv, jb);
+        if (va) force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
+        va = next(ja);
+        pa = load_pair(bv, ja);
+        if (vb) force_pair(bv, k, pb, xi, yi, zi, vxi, vyi, vzi, ai, a);
+    }
+}
+
+// s_waitcnt vmcnt(0) (gfx9 encoding; expcnt 7 and lgkmcnt 15: not waited on)
+constexpr unsigned kWaitVmcnt0 = 0x0F70;
+
+// The force sums of particle i from the row masks, as mask_sums, with another
+// division of the work: the wave reads the rows and words together (coalesced)
+// and each lane queues its non-empty words, without its own bit, in LDS (`qw`,
+// W entries at stride T); drain_words runs through every lane's queue when one
+// is full and at the end. With W large enough for a particle's words there is
+// one drain per wave and no per-bit fill rounds, at the price of more LDS.
+template <int T, int W>
+__device__ __forceinline__ ForceSums mask_sums_queue(const BufView &bv, const uint32_t *__restrict__ cell,
+                                                     const uint32_t *__restrict__ start,
+                                                     const uint64_t *__restrict__ masks, uint32_t *qw, uint32_t i,
+                                                     uint32_t li, const Scene &s, const Consts &k, float xi,
+                                                     float yi, float zi, float vxi, float vyi, float vzi, double ai)
+{
+    ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int x, y, z;
+    cell_xyz(cell[i], s, x, y, z);
+    const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
+    const uint32_t fallback = li + 1 < s.n ? li + 1 : li - 1;   // loaded by idle lanes, never added
+    int nq = 0;
+    for (int r = 0; r < 9; ++r) {
+        uint32_t jrun, jend;
+        row_run(rb, s, start, x, y, z, r, jrun, jend);
+        const uint32_t nw = (jend - jrun + 63u) >> 6;
+        for (uint32_t w = 0; __any(w < nw); ++w) {
+            uint64_t m = w < nw ? masks[mask_word(i, r, w)] : 0ull;
+            const uint32_t j0 = jrun + 64u * w, self = li - j0;
+            if (self < 64u) m &= ~(1ull << self);
+            if (m != 0ull) {
+                qw[(3 * nq) * T] = (uint32_t) m;
+                qw[(3 * nq + 1) * T] = (uint32_t) (m >> 32);
+                qw[(3 * nq + 2) * T] = j0;
+                ++nq;
+            }
+            if (__any(nq == W)) {
+                drain_words<T>(bv, k, qw, nq, fallback, xi, yi, zi, vxi, vyi, vzi, ai, a);
+                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);   // the drain's last, unused loads
+                nq = 0;
+            }
+        }
+    }
+    drain_words<T>(bv, k, qw, nq, fallback, xi, yi, zi, vxi, vyi, vzi, ai, a);
+    return a;
+}
+
 // (f_p + f_v + f_g) / rho from the force sums (src/CCPUParticleSimulator.cpp:155-193).
 __device__ __forceinline__ void accel_from_sums(const ForceSums &a, float rhoi, const Scene &s, const Consts &k,
                                                 float &ax, float &ay, float &az)
@@ -1097,7 +1181,7 @@ __global__ __launch_bounds__(256) void k_integrate(ParticleSoA p, const float *_
 // Neighbours: kNbWalk walks the cells; kNbLists takes the density pass's lists,
 // a wave at a time (wave_list_sums); kNbMasks iterates the density pass's row
 // masks (mask_sums). Particles whose list or masks overflowed walk their cells.
-template <int T, int U, int C, int kStall, int kMode, bool kStoreAcc>
+template <int T, int U, int C, int kStall, bool kQueue, int kMode, bool kStoreAcc>
 __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                     const uint32_t *__restrict__ start, DensityFields f,
                                                     NeighbourList nl, ParticleSoA out,
@@ -1139,7 +1223,10 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
                                s_acc + 128 * w, s_items + kLevelWindow * 64 * w);
         } else {
             overflow = cnt_i != 0;
-            if (!overflow)
+            if (!overflow && kQueue)
+                a = mask_sums_queue<T, C / 3>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
+                                              vxi, vyi, vzi, ai);
+            else if (!overflow)
                 a = mask_sums<T, C, kStall>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
                                             vxi, vyi, vzi, ai);
         }
@@ -1163,7 +1250,7 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
 inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 
 // The compiled shapes of the two neighbour kernels, indexed by variant.
-template <int T, int U, int C, int kStall>
+template <int T, int U, int C, int kStall, bool kQueue>
 struct NeighbourKernels {
     template <bool kLists, bool kMasks>
     static void density_mode(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
@@ -1194,10 +1281,10 @@ struct NeighbourKernels {
     {
         const dim3 g(blocks_for(t.nslots, T));
         if (ax)
-            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kMode, true>), g, dim3(T), 0, st, p, cell, start, f,
+            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kQueue, kMode, true>), g, dim3(T), 0, st, p, cell, start, f,
                                nl, out, key_next, ax, ay, az, t, k);
         else
-            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kMode, false>), g, dim3(T), 0, st, p, cell, start, f,
+            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kQueue, kMode, false>), g, dim3(T), 0, st, p, cell, start, f,
                                nl, out, key_next, ax, ay, az, t, k);
     }
     static void fused(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
@@ -1213,24 +1300,26 @@ struct NeighbourKernels {
     }
 };
 
-// X(index, threads, unroll, pair-list capacity and drain policy of the mask mode)
-#define SPH_VARIANTS(X)      \
-    X(0, 256, 4, 16, 0)      \
-    X(1, 128, 4, 16, 0)      \
-    X(2, 64, 4, 16, 0)       \
-    X(3, 256, 2, 16, 0)      \
-    X(4, 64, 8, 16, 0)       \
-    X(5, 64, 4, 32, 0)       \
-    X(6, 64, 8, 32, 0)       \
-    X(7, 64, 8, 32, 50)      \
-    X(8, 64, 8, 32, 75)      \
-    X(9, 64, 4, 32, 75)      \
-    X(10, 64, 8, 24, 75)     \
-    X(11, 64, 8, 64, 0)
+// X(index, threads, unroll, mask mode: LDS words per lane, stall drain policy, word queue)
+#define SPH_VARIANTS(X)         \
+    X(0, 256, 4, 16, 0, false)  \
+    X(1, 128, 4, 16, 0, false)  \
+    X(2, 64, 4, 16, 0, false)   \
+    X(3, 256, 2, 16, 0, false)  \
+    X(4, 64, 8, 16, 0, false)   \
+    X(5, 64, 4, 32, 0, false)   \
+    X(6, 64, 8, 32, 0, false)   \
+    X(7, 64, 8, 32, 50, false)  \
+    X(8, 64, 8, 32, 75, false)  \
+    X(9, 64, 4, 32, 75, false)  \
+    X(10, 64, 8, 24, 75, false) \
+    X(11, 64, 8, 64, 0, false)  \
+    X(12, 64, 8, 48, 0, true)   \
+    X(13, 64, 8, 64, 0, true)
 
 }  // namespace
 
-#define SPH_VARIANT_ROW(i, t, u, c, st) {t, u, c, st},
+#define SPH_VARIANT_ROW(i, t, u, c, st, q) {t, u, c, st, q ? 1 : 0},
 const NeighbourVariant kNeighbourVariants[kNumVariants] = {SPH_VARIANTS(SPH_VARIANT_ROW)};
 #undef SPH_VARIANT_ROW
 
@@ -1293,11 +1382,11 @@ void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *
 {
     if (t.nslots == 0) return;
     switch (variant) {
-#define SPH_DENSITY_CASE(i, t_, u, c, stl) \
-    case i: NeighbourKernels<t_, u, c, stl>::density(st, p, cell, start, f, nl, t, k); break;
+#define SPH_DENSITY_CASE(i, t_, u, c, stl, q) \
+    case i: NeighbourKernels<t_, u, c, stl, q>::density(st, p, cell, start, f, nl, t, k); break;
         SPH_VARIANTS(SPH_DENSITY_CASE)
 #undef SPH_DENSITY_CASE
-    default: NeighbourKernels<64, 8, 32, 0>::density(st, p, cell, start, f, nl, t, k); break;
+    default: NeighbourKernels<64, 8, 32, 0, false>::density(st, p, cell, start, f, nl, t, k); break;
     }
 }
 
@@ -1331,11 +1420,11 @@ void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint3
 {
     if (t.nslots == 0) return;
     switch (variant) {
-#define SPH_FUSED_CASE(i, t_, u, c, stl) \
-    case i: NeighbourKernels<t_, u, c, stl>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+#define SPH_FUSED_CASE(i, t_, u, c, stl, q) \
+    case i: NeighbourKernels<t_, u, c, stl, q>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
         SPH_VARIANTS(SPH_FUSED_CASE)
 #undef SPH_FUSED_CASE
-    default: NeighbourKernels<64, 8, 32, 0>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
+    default: NeighbourKernels<64, 8, 32, 0, false>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;
     }
 }
 

@@ -21,8 +21,8 @@ NEIGHBOUR_LISTS_OFF = 0xFFFFFFFF
  DEBUG_PRESSURE, DEBUG_ACCEL, DEBUG_IDS) = range(9)
 STAGES = ["hash", "sort", "cells", "reorder", "density", "forces", "collisions", "integrate"]
 EMIT_FOUNTAIN, EMIT_DISC = 0, 1
-NUM_VARIANTS = 12
-DEFAULT_VARIANT = 8
+NUM_VARIANTS = 14
+DEFAULT_VARIANT = 12
 NEIGHBOURS_WALK, NEIGHBOURS_LISTS, NEIGHBOURS_MASKS = 0, 1, 2
 GROUP_SLOTS = 256
 
@@ -67,7 +67,7 @@ SIGNATURES = {
     "sph_set_graphs": (C.c_int, [_vp, C.c_int]),
     "sph_set_variant": (C.c_int, [_vp, C.c_int, C.c_int]),
     "sph_get_variant": (C.c_int, [_vp, _vp, _vp]),
-    "sph_variant_shape": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),
+    "sph_variant_shape": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "sph_autotune": (C.c_int, [_vp, _u32, _vp]),
     "sph_slot_bytes": (C.c_uint64, [_u32]),
     "sph_scene_readback": (C.c_int, [_vp, _u32, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_u32)]),

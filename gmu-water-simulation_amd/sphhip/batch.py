@@ -196,8 +196,9 @@ class SceneBatch:
 
 
 def variant_shape(variant):
-    """(threads per workgroup, candidates per lane per trip, mask-mode pair list
-    capacity, mask-mode drain stall percent)."""
-    t, u, c, st = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
-    _lib.check(_lib.load().sph_variant_shape(int(variant), C.byref(t), C.byref(u), C.byref(c), C.byref(st)))
-    return t.value, u.value, c.value, st.value
+    """(threads per workgroup, candidates per lane per trip, mask-mode LDS words
+    per lane, mask-mode drain stall percent, mask-mode word queue 0/1)."""
+    t, u, c, st, q = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    _lib.check(_lib.load().sph_variant_shape(int(variant), C.byref(t), C.byref(u), C.byref(c), C.byref(st),
+                                             C.byref(q)))
+    return t.value, u.value, c.value, st.value, q.value
