@@ -47,7 +47,7 @@ sph_status fail(sph_status code, const std::string &msg)
     } while (0)
 
 constexpr int kProfileSlots = 256;          // event pairs buffered before a forced drain
-constexpr uint32_t kMaxSceneSlots = 1u << 26;  // 64-byte records of a scene stay below 4 GiB (buffer offsets)
+constexpr uint32_t kMaxSceneSlots = 1u << 26;  // 40-byte records of a scene stay below 4 GiB (buffer offsets)
 constexpr uint32_t kMaxPool = 1u << 31;
 constexpr uint32_t kNoScene = 0xFFFFFFFFu;
 
@@ -103,9 +103,7 @@ struct sph_ctx {
     uint32_t *cstart = nullptr;
 
     float *rho = nullptr;
-    double *prr = nullptr;
-    double *rinv = nullptr;
-    float4 *rec = nullptr;
+    uint32_t *rec = nullptr;
     float *acc[3] = {};
     uint32_t *ncount = nullptr;
     uint32_t *nlist = nullptr;   // neighbour lists, max_neighbours per slot, wave-blocked
@@ -146,8 +144,6 @@ struct sph_ctx {
     {
         sph::DensityFields f;
         f.rho = rho;
-        f.prr = prr;
-        f.rinv = rinv;
         f.ncount = diag ? ncount : nullptr;
         f.rec = rec;
         return f;
@@ -912,9 +908,7 @@ sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
     SPH_TRY(dalloc(c, &c->hist, hist_n));
     SPH_TRY(dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n)));
     SPH_TRY(dalloc(c, &c->rho, P));
-    SPH_TRY(dalloc(c, &c->prr, P));
-    SPH_TRY(dalloc(c, &c->rinv, P));
-    SPH_TRY(dalloc(c, &c->rec, 4 * P));
+    SPH_TRY(dalloc(c, &c->rec, (size_t) sph::kRecWords * P));
     for (int a = 0; a < 3; ++a) SPH_TRY(dalloc(c, &c->acc[a], P));
     SPH_TRY(dalloc(c, &c->ncount, P));
     SPH_TRY(dalloc(c, &c->nlist_cnt, P));
@@ -1090,11 +1084,11 @@ void sph_default_scene(sph_scene_desc *d, float box, uint32_t capacity)
 uint64_t sph_slot_bytes(uint32_t max_neighbours)
 {
     const uint64_t lists = max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0 : max_neighbours;
-    // pos/vel ping-pong 48, ids 8, keys and values 20, rho 4, p/rho^2 8, 1/rho 8,
-    // records 64, accelerations 12, counts 8, row masks 288 (the default mode
-    // without lists), list entries, and 4 for the histograms, the group map and
-    // the scene's share of the cell table
-    return 48 + 8 + 20 + 4 + 8 + 8 + 64 + 12 + 8 + (lists ? 4 * lists : 288) + 4;
+    // pos/vel ping-pong 48, ids 8, keys and values 20, rho 4, records 40,
+    // accelerations 12, counts 8, row masks 288 (the default mode without
+    // lists), list entries, and 4 for the histograms, the group map and the
+    // scene's share of the cell table
+    return 48 + 8 + 20 + 4 + 4 * sph::kRecWords + 12 + 8 + (lists ? 4 * lists : 288) + 4;
 }
 
 int sph_grid_res(float box)

@@ -18,13 +18,12 @@ struct ParticleSoA {
 // Per-particle results of the density pass that the force pass reads.
 struct DensityFields {
     float *rho;       // density
-    double *prr;      // p / rho^2 in double (src/CCPUParticleSimulator.cpp:181)
-    double *rinv;     // 1 / rho in double (exact-division helper of the force pass)
     uint32_t *ncount; // accepted neighbours incl. self (diagnostics; may be null)
-    // Everything the force pass reads about a neighbour, one 64-byte record per
-    // slot so that a pair costs three 16-byte loads from one cache line:
-    // {x, y, z, vx}, {vy, vz, -, -}, {p/rho^2, 1/rho} (doubles), 16 bytes padding.
-    float4 *rec;
+    // Everything the force pass reads about a neighbour, one 40-byte record per
+    // slot (kRecWords 32-bit words) read with three loads (16 + 16 + 8 bytes):
+    // x, y, z, vx, vy, vz, then as doubles p/rho^2 (src/CCPUParticleSimulator.cpp:181)
+    // and 1/rho (the exact-division helper of the force pass).
+    uint32_t *rec;
 };
 
 // Accepted neighbours of every particle except itself, in canonical order, as
@@ -38,6 +37,8 @@ struct DensityFields {
 // Or (mode kNbMasks) the density pass's per-row accepted-candidate masks
 // (`masks`, see the kernels), with cnt[i] = 1 for a particle they cannot hold.
 enum NeighbourMode : int { kNbWalk = 0, kNbLists = 1, kNbMasks = 2 };
+
+constexpr uint32_t kRecWords = 10;
 
 struct NeighbourList {
     uint32_t *idx;

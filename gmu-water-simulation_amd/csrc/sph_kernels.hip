@@ -449,7 +449,7 @@ __device__ __forceinline__ BufView buf_view(const ParticleSoA &p, const DensityF
     v.py = rsrc(p.py + s.poff, s.cap * 4u);
     v.pz = rsrc(p.pz + s.poff, s.cap * 4u);
     v.rho = rsrc(f.rho + s.poff, s.cap * 4u);
-    v.rec = rsrc(f.rec + 4 * (size_t) s.poff, s.cap * 64u);
+    v.rec = rsrc(f.rec + kRecWords * (size_t) s.poff, s.cap * (4u * kRecWords));
     return v;
 }
 
@@ -461,6 +461,11 @@ __device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, uint32_t byte_off
 __device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
 {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int) byte_off, 0, 0));
+}
+
+__device__ __forceinline__ uint2 ld2u(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, (int) byte_off, 0, 0));
 }
 
 // Squared lower bounds on the distance from a particle to any particle of the
@@ -684,14 +689,14 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
     // rho*rho is exact in double, so this is p / pow(rho, 2)
     const double prr = (double) pressure / ((double) density * (double) density);
     const double rinv = 1.0 / (double) density;
-    f.prr[i] = prr;
-    f.rinv[i] = rinv;
-    float4 *rec = f.rec + 4 * (size_t) i;
-    rec[0] = make_float4(xi, yi, zi, p.vx[i]);
-    rec[1] = make_float4(p.vy[i], p.vz[i], 0.0f, 0.0f);
+    // the record, as five 8-byte stores (records are 8-byte aligned)
+    uint2 *rec = reinterpret_cast<uint2 *>(f.rec + kRecWords * (size_t) i);
     const uint64_t pb = __builtin_bit_cast(uint64_t, prr), rb = __builtin_bit_cast(uint64_t, rinv);
-    rec[2] = make_float4(__uint_as_float((uint32_t) pb), __uint_as_float((uint32_t) (pb >> 32)),
-                         __uint_as_float((uint32_t) rb), __uint_as_float((uint32_t) (rb >> 32)));
+    rec[0] = make_uint2(__float_as_uint(xi), __float_as_uint(yi));
+    rec[1] = make_uint2(__float_as_uint(zi), __float_as_uint(p.vx[i]));
+    rec[2] = make_uint2(__float_as_uint(p.vy[i]), __float_as_uint(p.vz[i]));
+    rec[3] = make_uint2((uint32_t) pb, (uint32_t) (pb >> 32));
+    rec[4] = make_uint2((uint32_t) rb, (uint32_t) (rb >> 32));
     if (kLists) nl.cnt[i] = nn;
     if (kMasks) nl.cnt[i] = rec_masks.overflow ? 1u : 0u;
     if (kDiag) f.ncount[i] = nc;
@@ -709,10 +714,18 @@ struct PairData {
     double prr, rinv;
 };
 
+// p/rho^2 of slot i from its record.
+__device__ __forceinline__ double own_prr(const DensityFields &f, uint32_t i)
+{
+    const uint2 w = *reinterpret_cast<const uint2 *>(f.rec + kRecWords * (size_t) i + 6);
+    return __builtin_bit_cast(double, (uint64_t) w.x | ((uint64_t) w.y << 32));
+}
+
 __device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
 {
-    const uint32_t o = j * 64u;
-    const float4 a = ld4(bv.rec, o), b = ld4(bv.rec, o + 16u), c = ld4(bv.rec, o + 32u);
+    const uint32_t o = j * (4u * kRecWords);
+    const float4 a = ld4(bv.rec, o), b = ld4(bv.rec, o + 16u);
+    const uint2 c = ld2u(bv.rec, o + 32u);
     PairData d;
     d.j = j;
     d.x = a.x;
@@ -721,8 +734,8 @@ __device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
     d.vx = a.w;
     d.vy = b.x;
     d.vz = b.y;
-    d.prr = __builtin_bit_cast(double, (uint64_t) __float_as_uint(c.x) | ((uint64_t) __float_as_uint(c.y) << 32));
-    d.rinv = __builtin_bit_cast(double, (uint64_t) __float_as_uint(c.z) | ((uint64_t) __float_as_uint(c.w) << 32));
+    d.prr = __builtin_bit_cast(double, (uint64_t) __float_as_uint(b.z) | ((uint64_t) __float_as_uint(b.w) << 32));
+    d.rinv = __builtin_bit_cast(double, (uint64_t) c.x | ((uint64_t) c.y << 32));
     return d;
 }
 
@@ -1004,7 +1017,7 @@ __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *
 {
     const BufView bv = buf_view(p, f, s);
     const float rhoi = f.rho[i];
-    const double ai = f.prr[i];
+    const double ai = own_prr(f, i);
     ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t cnt_i = nl.mode == kNbLists ? nl.cnt[i] : 0u;
     if (nl.mode == kNbLists && cnt_i <= nl.kmax) {
@@ -1212,7 +1225,7 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
     } else {
         const BufView bv = buf_view(p, f, s);
         const float rhoi = f.rho[i];
-        const double ai = f.prr[i];
+        const double ai = own_prr(f, i);
         const uint32_t cnt_i = nl.cnt[i];
         ForceSums a;
         bool overflow;
