@@ -361,12 +361,40 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
 }
 
+// start[c] = the first sorted slot whose key is >= c (lower bound), for every
+// c in [0, ncells]. The previous step's start[c] is the starting guess: particles
+// move little per step, so the bound is usually there or a few slots away; from
+// a wrong guess (any value, e.g. after a layout change) an exponential search
+// and a binary search still find it. The result does not depend on the guess.
 __global__ void k_cell_start(const uint32_t *__restrict__ keys, uint32_t n, uint32_t ncells,
                              uint32_t *__restrict__ start)
 {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > ncells) return;
-    uint32_t lo = 0, hi = n;
+    uint32_t h = start[c];
+    if (h > n) h = n;
+    uint32_t lo, hi;
+    if (h < n && keys[h] < c) {   // the bound is right of h: keys[a - 1] < c
+        uint32_t a = h + 1, b = a, step = 1;
+        while (b < n && keys[b] < c) {
+            a = b + 1;
+            b = a + step;
+            step <<= 1;
+        }
+        lo = a;
+        hi = b < n ? b : n;
+    } else if (h > 0 && keys[h - 1] >= c) {   // left of h: keys[b] >= c
+        uint32_t b = h - 1, a = b, step = 1;
+        while (a > 0 && keys[a - 1] >= c) {
+            b = a - 1;
+            a = b >= step ? b - step : 0;
+            step <<= 1;
+        }
+        lo = a;
+        hi = b;
+    } else {
+        lo = hi = h;
+    }
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (keys[mid] < c) lo = mid + 1; else hi = mid;
