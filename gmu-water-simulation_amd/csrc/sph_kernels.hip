@@ -569,8 +569,9 @@ struct NoRowHooks {
 // candidate, and appends without branching (writes slot cnt, then adds the
 // accept bit), so a list keeps U slots of headroom; when any lane's list is that
 // full, flush() drains every lane's list. hooks.trip(j0, bits) gets the accept
-// bits of each trip (bit u: candidate j0 + u).
-template <int U, typename Test, typename Flush, typename Hooks = NoRowHooks>
+// bits of each trip (bit u: candidate j0 + u). kWholeTrips: a trip in which
+// every lane has U candidates left skips the per-candidate run-end test.
+template <int U, bool kWholeTrips = false, typename Test, typename Flush, typename Hooks = NoRowHooks>
 __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k, const uint32_t *__restrict__ start,
                                                 const BufView &bv, int x, int y, int z, float xi, float yi,
                                                 float zi, int &cnt, Test test, Flush flush, Hooks &&hooks = Hooks())
@@ -592,9 +593,15 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
                     cz[u] = ldf(bv.pz, off + 4u * u);
                 }
                 uint32_t bits = 0;
+                if (kWholeTrips && __all(rem >= (uint32_t) U)) {
+                    // every trip of the wave is whole: no per-candidate run-end test
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    bits |= (uint32_t) test(j + u, (uint32_t) u < rem, cx[u], cy[u], cz[u]) << u;
+                    for (int u = 0; u < U; ++u) bits |= (uint32_t) test(j + u, true, cx[u], cy[u], cz[u]) << u;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        bits |= (uint32_t) test(j + u, (uint32_t) u < rem, cx[u], cy[u], cz[u]) << u;
+                }
                 hooks.trip(j, bits);
                 j += U;
             }
@@ -848,7 +855,7 @@ __device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t
     int x, y, z;
     cell_xyz(cell[i], s, x, y, z);
     int cnt = 0;
-    walk_neighbours<U>(
+    walk_neighbours<U, true>(
         s, k, start, bv, x, y, z, xi, yi, zi, cnt,
         [&](uint32_t j, bool valid, float xj, float yj, float zj) {
             const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
@@ -1356,7 +1363,8 @@ struct NeighbourKernels {
     X(10, 64, 8, 24, 75, false) \
     X(11, 64, 8, 64, 0, false)  \
     X(12, 64, 8, 48, 0, true)   \
-    X(13, 64, 8, 64, 0, true)
+    X(13, 64, 8, 64, 0, true)   \
+    X(14, 64, 4, 48, 0, true)
 
 }  // namespace
 
