@@ -109,7 +109,7 @@ struct sph_ctx {
     uint32_t *nlist = nullptr;   // neighbour lists, max_neighbours per slot, wave-blocked
     uint32_t *nlist_cnt = nullptr;
     uint32_t kmax = 0;           // list capacity allocated
-    uint64_t *masks = nullptr;   // row masks (mode SPH_NEIGHBOURS_MASKS), allocated on first use
+    uint32_t *masks = nullptr;   // row masks (mode SPH_NEIGHBOURS_MASKS), allocated on first use
     int nb_mode = SPH_NEIGHBOURS_WALK;
     int variant = sph::kDefaultVariant;
 
@@ -981,7 +981,7 @@ sph_status create_ctx(uint32_t pool, uint32_t max_scenes, uint32_t flags, uint32
         return fail(SPH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     sph_status st = alloc_buffers(c, 2 * max_scenes + 1);
-    if (!st && c->nb_mode == SPH_NEIGHBOURS_MASKS) st = dalloc(c, &c->masks, (size_t) 9 * 4 * round_up(c->pool, 64));
+    if (!st && c->nb_mode == SPH_NEIGHBOURS_MASKS) st = dalloc(c, &c->masks, (size_t) 9 * 8 * round_up(c->pool, 64));
     if (!st) st = rebuild_layout(c);
     if (st) {
         std::string msg = g_last_error;
@@ -1392,7 +1392,7 @@ sph_status ensure_masks(sph_ctx *c)
 {
     if (c->masks) return SPH_OK;
     SPH_HIP(hipStreamSynchronize(c->stream));
-    return dalloc(c, &c->masks, (size_t) 9 * 4 * round_up(c->pool, 64));
+    return dalloc(c, &c->masks, (size_t) 9 * 8 * round_up(c->pool, 64));
 }
 
 sph_status sph_set_variant(sph_ctx *c, int variant, int mode)

@@ -45,7 +45,7 @@ struct NeighbourList {
     uint32_t *cnt;
     uint32_t kmax;
     int mode;            // NeighbourMode
-    uint64_t *masks;
+    uint32_t *masks;     // mask mode: 32-bit words of accepted-candidate bits
 };
 
 __host__ __device__ inline size_t list_slot(const NeighbourList &nl, uint32_t i)
@@ -67,7 +67,7 @@ inline uint32_t scan_blocks(uint32_t m) { return (m + kScanTile - 1) / kScanTile
 // tested per lane per trip, and how the force pass reads the row masks: with
 // `pairs` 32-bit LDS words per lane either as a pair list with a drain policy
 // (stall 0: drain when any list is full; P: full lanes wait until P percent of
-// the wave waits) or, with `queue`, as a queue of pairs / 3 mask words.
+// the wave waits) or, with `queue`, as a queue of pairs / 2 mask words.
 // Results do not depend on the variant.
 struct NeighbourVariant {
     int threads;
@@ -76,7 +76,7 @@ struct NeighbourVariant {
     int stall;
     int queue;
 };
-constexpr int kNumVariants = 15;
+constexpr int kNumVariants = 17;
 constexpr int kDefaultVariant = 12;  // 64 threads, 8 candidates, a 16-word mask queue: fastest measured on the 1M dam break
 extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 

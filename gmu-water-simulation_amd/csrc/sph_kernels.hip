@@ -617,26 +617,29 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
 }
 
 // Accepted-candidate masks of the density pass, for the force pass: bit b of
-// row r's mask is candidate (run start + b) of that row. A row holds up to
-// kMaskBits candidates; a particle with a longer run has no masks (its force
-// pass walks). Stored word-major per wave: word w of row r of slot i is
-// masks[(((i / 64) * 9 + r) * kMaskWords + w) * 64 + i % 64], so the lanes of a
-// wave write and read each word with one coalesced access.
-constexpr uint32_t kMaskWords = 4;
-constexpr uint32_t kMaskBits = 64 * kMaskWords;
+// row r's mask is candidate (run start + b) of that row, kept in 32-bit words
+// (a run of up to 32 candidates, the common case on sparse scenes, costs 4
+// bytes). A row holds up to kMaskBits candidates; a particle with a longer run
+// has no masks (its force pass walks). Stored word-major per wave: word w of row
+// r of slot i is masks[(((i / 64) * 9 + r) * kMaskWords + w) * 64 + i % 64], so
+// the lanes of a wave write and read each word with one coalesced access.
+constexpr uint32_t kMaskWordBits = 32;
+constexpr uint32_t kMaskWords = 8;
+constexpr uint32_t kMaskBits = kMaskWordBits * kMaskWords;
 
 __device__ __forceinline__ size_t mask_word(uint32_t i, int r, uint32_t w)
 {
     return ((size_t) ((i >> 6) * 9u + (uint32_t) r) * kMaskWords + w) * 64u + (i & 63u);
 }
 
-// Records the accept bits of each trip into the row's current 64-bit word,
-// storing a word when the run moves past it and at the end of the row.
+// Records the accept bits of each trip into the row's current word (a trip of
+// U candidates never straddles two words: U divides 32), storing a word when the
+// run moves past it and at the end of the row.
 struct MaskRecorder {
-    uint64_t *masks;
+    uint32_t *masks;
     uint32_t i;
     uint32_t jrow = 0, len = 0, word = 0;
-    uint64_t cur = 0;
+    uint32_t cur = 0;
     bool overflow = false;
     int row = 0;
 
@@ -651,13 +654,13 @@ struct MaskRecorder {
     }
     __device__ __forceinline__ void trip(uint32_t j0, uint32_t bits)
     {
-        const uint32_t b = j0 - jrow, w = b >> 6;
+        const uint32_t b = j0 - jrow, w = b / kMaskWordBits;
         if (w != word) {   // runs visit their candidates in order: word only grows
             if (word < kMaskWords) masks[mask_word(i, row, word)] = cur;
             word = w;
             cur = 0;
         }
-        cur |= (uint64_t) bits << (b & 63u);
+        cur |= bits << (b % kMaskWordBits);
     }
     __device__ __forceinline__ void row_end(int)
     {
@@ -674,9 +677,10 @@ struct MaskRecorder {
 template <int T, int U, bool kLists, bool kMasks, bool kDiag>
 __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                const uint32_t *__restrict__ start, DensityFields f,
-                                               NeighbourList nl, uint64_t *__restrict__ masks, SceneTable t,
+                                               NeighbourList nl, uint32_t *__restrict__ masks, SceneTable t,
                                                Consts k)
 {
+    static_assert(kMaskWordBits % U == 0, "a trip must not straddle two mask words");
     static_assert(kGroupSlots % T == 0, "a workgroup must lie in one slot group");
     __shared__ float lst[kListCap * T];
     const uint32_t first = xcd_block(blockIdx.x, gridDim.x) * (uint32_t) T, i = first + threadIdx.x;
@@ -880,7 +884,7 @@ __device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t
 template <int T, int C, int kStall>
 __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t *__restrict__ cell,
                                                const uint32_t *__restrict__ start,
-                                               const uint64_t *__restrict__ masks, uint32_t *my, uint32_t i,
+                                               const uint32_t *__restrict__ masks, uint32_t *my, uint32_t i,
                                                uint32_t li, const Scene &s, const Consts &k, float xi, float yi,
                                                float zi, float vxi, float vyi, float vzi, double ai)
 {
@@ -892,13 +896,13 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
     // lane's unread bits of word w of row r, whose first candidate is jw
     int r = -1;
     uint32_t w = 0, nw = 0, jrun = 0, jw = 0;
-    uint64_t m = 0;
+    uint32_t m = 0;
     bool done = false;
     int cnt = 0;
     for (;;) {
         // fill the pair lists until a drain is due; the sums are not touched here
         for (;;) {
-            if (!__any(m != 0ull)) {
+            if (!__any(m != 0u)) {
                 ++w;
                 if (!__any(w < nw)) {
                     if (++r == 9) {
@@ -907,20 +911,20 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
                     }
                     uint32_t jend;
                     row_run(rb, s, start, x, y, z, r, jrun, jend);
-                    nw = (jend - jrun + 63u) >> 6;
+                    nw = (jend - jrun + kMaskWordBits - 1) / kMaskWordBits;
                     w = 0;
                 }
-                m = w < nw ? masks[mask_word(i, r, w)] : 0ull;
-                jw = jrun + 64u * w;
+                m = w < nw ? masks[mask_word(i, r, w)] : 0u;
+                jw = jrun + kMaskWordBits * w;
                 continue;
             }
             // two bits per round; kStall: a lane whose list is full keeps its
             // bits until the drain (without kStall a list has room for both)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                if (m != 0ull && (kStall == 0 || cnt < C)) {
-                    const uint32_t jj = jw + (uint32_t) __builtin_ctzll(m);
-                    m &= m - 1ull;
+                if (m != 0u && (kStall == 0 || cnt < C)) {
+                    const uint32_t jj = jw + (uint32_t) __builtin_ctz(m);
+                    m &= m - 1u;
                     my[cnt * T] = jj;
                     cnt += jj != li;
                 }
@@ -932,7 +936,7 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
                 const bool full = cnt == C;
                 const uint32_t waiting = (uint32_t) __popcll(__ballot(full));
                 const uint32_t lanes = (uint32_t) __popcll(__ballot(true));
-                drain = waiting * 100u >= (uint32_t) kStall * lanes || (waiting && !__any(m != 0ull && !full));
+                drain = waiting * 100u >= (uint32_t) kStall * lanes || (waiting && !__any(m != 0u && !full));
             }
             if (drain) break;
         }
@@ -943,28 +947,26 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
     return a;
 }
 
-// Drains a lane's queue of mask words (nq entries of three 32-bit words at
-// stride T: bits 0-31, bits 32-63, slot of bit 0) into its force sums: the set
-// bits of each word in ascending order, word after word. Two pairs are in
-// flight, as in drain_pairs; a lane with no pair left loads `fallback` and adds
-// nothing.
+// Drains a lane's queue of mask words (nq entries of two 32-bit words at
+// stride T: the bits and the slot of bit 0) into its force sums: the set bits of
+// each word in ascending order, word after word. Two pairs are in flight, as in
+// drain_pairs; a lane with no pair left loads `fallback` and adds nothing.
 template <int T>
 __device__ __forceinline__ void drain_words(const BufView &bv, const Consts &k, const uint32_t *qw, int nq,
                                             uint32_t fallback, float xi, float yi, float zi, float vxi, float vyi,
                                             float vzi, double ai, ForceSums &a)
 {
     int q = 0;
-    uint64_t m = 0;
-    uint32_t j0 = 0;
+    uint32_t m = 0, j0 = 0;
     auto next = [&](uint32_t &j) {
-        if (m == 0ull && q < nq) {
-            m = (uint64_t) qw[(3 * q) * T] | ((uint64_t) qw[(3 * q + 1) * T] << 32);
-            j0 = qw[(3 * q + 2) * T];
+        if (m == 0u && q < nq) {
+            m = qw[(2 * q) * T];
+            j0 = qw[(2 * q + 1) * T];
             ++q;
         }
-        const bool ok = m != 0ull;
-        j = ok ? j0 + (uint32_t) __builtin_ctzll(m) : fallback;
-        m &= m - 1ull;
+        const bool ok = m != 0u;
+        j = ok ? j0 + (uint32_t) __builtin_ctz(m) : fallback;
+        m &= m - 1u;
         return ok;
     };
     uint32_t ja, jb;
@@ -992,7 +994,7 @@ constexpr unsigned kWaitVmcnt0 = 0x0F70;
 template <int T, int W>
 __device__ __forceinline__ ForceSums mask_sums_queue(const BufView &bv, const uint32_t *__restrict__ cell,
                                                      const uint32_t *__restrict__ start,
-                                                     const uint64_t *__restrict__ masks, uint32_t *qw, uint32_t i,
+                                                     const uint32_t *__restrict__ masks, uint32_t *qw, uint32_t i,
                                                      uint32_t li, const Scene &s, const Consts &k, float xi,
                                                      float yi, float zi, float vxi, float vyi, float vzi, double ai)
 {
@@ -1001,19 +1003,22 @@ __device__ __forceinline__ ForceSums mask_sums_queue(const BufView &bv, const ui
     cell_xyz(cell[i], s, x, y, z);
     const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
     const uint32_t fallback = li + 1 < s.n ? li + 1 : li - 1;   // loaded by idle lanes, never added
+    // the first word of every row, loaded together up front (unused where a row is empty)
+    uint32_t m0[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) m0[r] = masks[mask_word(i, r, 0)];
     int nq = 0;
     for (int r = 0; r < 9; ++r) {
         uint32_t jrun, jend;
         row_run(rb, s, start, x, y, z, r, jrun, jend);
-        const uint32_t nw = (jend - jrun + 63u) >> 6;
+        const uint32_t nw = (jend - jrun + kMaskWordBits - 1) / kMaskWordBits;
         for (uint32_t w = 0; __any(w < nw); ++w) {
-            uint64_t m = w < nw ? masks[mask_word(i, r, w)] : 0ull;
-            const uint32_t j0 = jrun + 64u * w, self = li - j0;
-            if (self < 64u) m &= ~(1ull << self);
-            if (m != 0ull) {
-                qw[(3 * nq) * T] = (uint32_t) m;
-                qw[(3 * nq + 1) * T] = (uint32_t) (m >> 32);
-                qw[(3 * nq + 2) * T] = j0;
+            uint32_t m = w >= nw ? 0u : w == 0 ? m0[r] : masks[mask_word(i, r, w)];
+            const uint32_t j0 = jrun + kMaskWordBits * w, self = li - j0;
+            if (self < kMaskWordBits) m &= ~(1u << self);
+            if (m != 0u) {
+                qw[(2 * nq) * T] = m;
+                qw[(2 * nq + 1) * T] = j0;
                 ++nq;
             }
             if (__any(nq == W)) {
@@ -1272,7 +1277,7 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
         } else {
             overflow = cnt_i != 0;
             if (!overflow && kQueue)
-                a = mask_sums_queue<T, C / 3>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
+                a = mask_sums_queue<T, C / 2>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
                                               vxi, vyi, vzi, ai);
             else if (!overflow)
                 a = mask_sums<T, C, kStall>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
@@ -1364,7 +1369,9 @@ struct NeighbourKernels {
     X(11, 64, 8, 64, 0, false)  \
     X(12, 64, 8, 48, 0, true)   \
     X(13, 64, 8, 64, 0, true)   \
-    X(14, 64, 4, 48, 0, true)
+    X(14, 64, 4, 48, 0, true)   \
+    X(15, 64, 8, 32, 0, true)   \
+    X(16, 64, 4, 32, 0, true)
 
 }  // namespace
 

@@ -287,14 +287,14 @@ sph_status sph_set_graphs(sph_ctx *ctx, int enable);
  *                        kept (288 bytes per slot).
  * A new context uses variant 8 and SPH_NEIGHBOURS_MASKS, or SPH_NEIGHBOURS_LISTS when it
  * was created with max_neighbours > 0. Results are identical for every choice. */
-#define SPH_NUM_VARIANTS 15
+#define SPH_NUM_VARIANTS 17
 enum { SPH_NEIGHBOURS_WALK = 0, SPH_NEIGHBOURS_LISTS = 1, SPH_NEIGHBOURS_MASKS = 2 };
 sph_status sph_set_variant(sph_ctx *ctx, int variant, int mode);
 sph_status sph_get_variant(const sph_ctx *ctx, int *variant, int *mode);
 /* Threads per workgroup, candidates tested per lane per trip, and how the mask mode's
  * force pass reads the masks, of a variant: with `pairs` 32-bit LDS words per lane, either
  * a pair list (stall 0 = drain when one list is full, P = full lanes wait until P percent
- * of the wave waits) or, queue = 1, a queue of pairs / 3 mask words drained in one pass.
+ * of the wave waits) or, queue = 1, a queue of pairs / 2 mask words drained in one pass.
  * Any pointer may be NULL. */
 sph_status sph_variant_shape(int variant, int *threads, int *unroll, int *pairs, int *stall, int *queue);
 
