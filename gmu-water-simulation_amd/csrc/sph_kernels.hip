@@ -310,31 +310,41 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data
     }
 }
 
-// Stable scatter of one LSD radix pass. Items of a tile are taken in slot order
-// (round k covers slots tile + k*256 .. +255); equal digits inside a wave are
-// matched with 8 ballots, ranked by popcount of the lower lanes, and offset by
-// the counts of lower waves and earlier rounds.
+// Stable scatter of one LSD radix pass. The tile's items (slot order: round k
+// covers slots tile + k*256 .. +255) are all loaded up front; equal digits
+// inside a wave are matched with 8 ballots and ranked by popcount of the lower
+// lanes. Each (round, wave) records its digit counts in LDS; one exclusive scan
+// per digit over the tile's (round, wave) sequence then gives every item its
+// offset among the equal digits before it, with two barriers per tile.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, uint32_t n, int shift, const uint32_t *__restrict__ hist, uint32_t nblocks)
 {
     constexpr int W = kSortThreads / 64;
+    static_assert(kRadixBuckets == kSortThreads, "one thread scans one digit");
+    static_assert(kSortTile <= 65535, "tile offsets are 16-bit");
     __shared__ uint32_t base_s[kRadixBuckets];
-    __shared__ uint32_t wcnt[W][kRadixBuckets];
+    __shared__ uint16_t cnt[kSortItems * W][kRadixBuckets];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    for (int d = t; d < kRadixBuckets; d += kSortThreads) base_s[d] = hist[(uint32_t) d * nblocks + blockIdx.x];
+    base_s[t] = hist[(uint32_t) t * nblocks + blockIdx.x];
+    {
+        uint32_t *c32 = reinterpret_cast<uint32_t *>(&cnt[0][0]);
+        for (int q = t; q < kSortItems * W * kRadixBuckets / 2; q += kSortThreads) c32[q] = 0u;
+    }
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t tile = blockIdx.x * (uint32_t) kSortTile;
-    for (int k = 0; k < kSortItems; ++k) {
-        for (int d = t; d < kRadixBuckets; d += kSortThreads)
+    uint32_t keys[kSortItems], vals[kSortItems], rank[kSortItems];
 #pragma unroll
-            for (int ww = 0; ww < W; ++ww) wcnt[ww][d] = 0;
-        __syncthreads();
+    for (int k = 0; k < kSortItems; ++k) {
         const uint32_t idx = tile + (uint32_t) (k * kSortThreads + t);
-        const bool valid = idx < n;
-        const uint32_t key = valid ? keys_in[idx] : 0u;
-        const uint32_t val = valid ? (vals_in ? vals_in[idx] : idx) : 0u;
-        const uint32_t dig = (key >> shift) & (kRadixBuckets - 1);
+        keys[k] = idx < n ? keys_in[idx] : 0u;
+        vals[k] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const bool valid = tile + (uint32_t) (k * kSortThreads + t) < n;
+        const uint32_t dig = (keys[k] >> shift) & (kRadixBuckets - 1);
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
@@ -342,21 +352,27 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
-        const uint32_t rank = (uint32_t) __popcll(m & below);
-        if (valid && rank == 0) wcnt[w][dig] = (uint32_t) __popcll(m);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = base_s[dig] + rank;
-            for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][dig];
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+        rank[k] = (uint32_t) __popcll(m & below);
+        if (valid && rank[k] == 0) cnt[k * W + w][dig] = (uint16_t) __popcll(m);
+    }
+    __syncthreads();
+    {
+        uint32_t run = 0;
+#pragma unroll 16
+        for (int q = 0; q < kSortItems * W; ++q) {
+            const uint32_t v = cnt[q][t];
+            cnt[q][t] = (uint16_t) run;
+            run += v;
         }
-        __syncthreads();
-        for (int d = t; d < kRadixBuckets; d += kSortThreads) {
-            uint32_t add = 0;
+    }
+    __syncthreads();
 #pragma unroll
-            for (int ww = 0; ww < W; ++ww) add += wcnt[ww][d];
-            base_s[d] += add;
+    for (int k = 0; k < kSortItems; ++k) {
+        if (tile + (uint32_t) (k * kSortThreads + t) < n) {
+            const uint32_t dig = (keys[k] >> shift) & (kRadixBuckets - 1);
+            const uint32_t pos = base_s[dig] + cnt[k * W + w][dig] + rank[k];
+            keys_out[pos] = keys[k];
+            vals_out[pos] = vals[k];
         }
     }
 }
