@@ -81,6 +81,12 @@ struct sph_ctx {
     std::vector<sph::Emitter> emit_h;        // emitters with their segment index
     sph::Scene *table_d = nullptr;
     uint32_t *group_d = nullptr;
+    // radix-sort tiles over the segments below nslots, and the bits of the
+    // largest segment-local key
+    std::vector<sph::SortTile> tiles_h;
+    sph::SortTile *tiles_d = nullptr;
+    uint32_t tiles_cap = 0;
+    int sort_bits = 1;
     sph::Emitter *emit_d = nullptr;
     uint32_t total_cells = 0;                // cells of all segments (grid cells + one dead cell each)
     uint32_t cstart_cap = 0;                 // entries allocated in cstart
@@ -442,6 +448,32 @@ sph_status rebuild_layout(sph_ctx *c)
         while (t + 1 < c->table_h.size() && c->table_h[t].poff + c->table_h[t].cap <= slot) ++t;
         c->group_h[g] = (uint32_t) t;
     }
+    // sort tiles: runs of groups of one segment, cut every kSortTile slots
+    c->tiles_h.clear();
+    c->sort_bits = 1;
+    for (uint32_t g0 = 0; g0 < ngroups;) {
+        uint32_t g1 = g0 + 1;
+        while (g1 < ngroups && c->group_h[g1] == c->group_h[g0]) ++g1;
+        const sph::Scene &seg = c->table_h[c->group_h[g0]];
+        const uint32_t a = g0 * sph::kGroupSlots;
+        const uint32_t b = std::min<uint64_t>((uint64_t) g1 * sph::kGroupSlots, c->nslots);
+        const uint32_t first = (uint32_t) c->tiles_h.size(), count = (b - a + sph::kSortTile - 1) / sph::kSortTile;
+        for (uint32_t k = 0; k < count; ++k) {
+            sph::SortTile tl{};
+            tl.start = a + k * (uint32_t) sph::kSortTile;
+            tl.count = std::min<uint32_t>((uint32_t) sph::kSortTile, b - tl.start);
+            tl.coff = seg.coff;
+            tl.hbase = first * (uint32_t) sph::kRadixBuckets + k;
+            tl.stride = count;
+            c->tiles_h.push_back(tl);
+        }
+        c->sort_bits = std::max(c->sort_bits, key_bits(seg.ncells + 1));
+        g0 = g1;
+    }
+    if (c->tiles_h.size() > c->tiles_cap) return fail(SPH_ERR_CAPACITY, "sort tiles exceed their buffer");
+    if (!c->tiles_h.empty())
+        SPH_HIP(hipMemcpy(c->tiles_d, c->tiles_h.data(), c->tiles_h.size() * sizeof(sph::SortTile),
+                          hipMemcpyHostToDevice));
     if (c->total_cells + 1 > c->cstart_cap) {
         dfree(c, c->cstart);
         c->cstart = nullptr;
@@ -547,15 +579,15 @@ sph_status run_grid(sph_ctx *c)
     {
         StageTimer t(c, SPH_STAGE_SORT);
         if (t.st) return t.st;
-        const int bits = key_bits(c->total_cells);
-        const int passes = (bits + sph::kRadixBits - 1) / sph::kRadixBits;
+        const int passes = (c->sort_bits + sph::kRadixBits - 1) / sph::kRadixBits;
         const uint32_t *kin = c->key_h;
         const uint32_t *vin = nullptr;
         for (int p = 0; p < passes; ++p) {
             const bool to_s = ((passes - 1 - p) % 2) == 0;
             uint32_t *kout = to_s ? c->key_s : c->key_t;
             uint32_t *vout = to_s ? c->val_s : c->val_t;
-            sph::launch_radix_pass(kin, vin, kout, vout, n, p * sph::kRadixBits, c->hist, c->partials, c->stream);
+            sph::launch_radix_pass(kin, vin, kout, vout, c->tiles_d, (uint32_t) c->tiles_h.size(), p * sph::kRadixBits,
+                                   c->hist, c->partials, c->stream);
             kin = kout;
             vin = vout;
         }
@@ -904,9 +936,11 @@ sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
     SPH_TRY(dalloc(c, &c->key_s, P));
     SPH_TRY(dalloc(c, &c->val_t, P));
     SPH_TRY(dalloc(c, &c->val_s, P));
-    const size_t hist_n = (size_t) sph::kRadixBuckets * sph::sort_blocks(c->pool);
+    c->tiles_cap = sph::max_sort_tiles(c->pool, max_segments);
+    const size_t hist_n = (size_t) sph::kRadixBuckets * c->tiles_cap;
     SPH_TRY(dalloc(c, &c->hist, hist_n));
     SPH_TRY(dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n)));
+    SPH_TRY(dalloc(c, &c->tiles_d, c->tiles_cap));
     SPH_TRY(dalloc(c, &c->rho, P));
     SPH_TRY(dalloc(c, &c->rec, (size_t) sph::kRecWords * P));
     for (int a = 0; a < 3; ++a) SPH_TRY(dalloc(c, &c->acc[a], P));

@@ -89,12 +89,28 @@ void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleS
 // for live particles, the scene's dead cell for unused slots.
 void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, double h, hipStream_t st);
 
-// One stable LSD radix pass over 8 bits starting at `shift`. vals_in == nullptr
-// means the identity (val = slot). hist holds kRadixBuckets * sort_blocks(n)
-// entries, partials scan_blocks(kRadixBuckets * sort_blocks(n)) entries.
+// A tile of the radix sort: up to kSortTile slots [start, start + count) of one
+// segment (a scene or a free range) whose cells start at coff. The digits are
+// those of the segment-local key (key - coff), and the tile's digit counts go
+// to hist[hbase + d * stride], stride = the segment's tile count, so the
+// histograms are ordered segment, digit, tile: their exclusive scan places each
+// segment's items in the segment's own slots, sorted by local key, which is
+// the order of a sort of the global keys (segments never share a slot, and the
+// segments' cells come in slot order).
+struct SortTile {
+    uint32_t start, count, coff, hbase, stride, pad[3];
+};
+
+// Tiles of kSortTile slots over segments; returns the tile count (at most
+// sort_blocks(nslots) + the number of segments).
+inline uint32_t max_sort_tiles(uint32_t pool, uint32_t max_segments) { return sort_blocks(pool) + max_segments; }
+
+// One stable LSD radix pass over 8 bits of the segment-local keys starting at
+// `shift`. vals_in == nullptr means the identity (val = slot). hist holds
+// kRadixBuckets * ntiles entries, partials scan_blocks(kRadixBuckets * ntiles).
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
-                       uint32_t *vals_out, uint32_t n, int shift, uint32_t *hist, uint32_t *partials,
-                       hipStream_t st);
+                       uint32_t *vals_out, const SortTile *tiles, uint32_t ntiles, int shift, uint32_t *hist,
+                       uint32_t *partials, hipStream_t st);
 
 // start[c] = first slot whose sorted key is >= c, c in [0, ncells].
 void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells, uint32_t *start,

@@ -209,22 +209,22 @@ __global__ __launch_bounds__(256) void k_emit(const Emitter *__restrict__ em, Sc
 }
 
 // Digit histogram of one tile, stored digit-major: hist[d * nblocks + block].
-__global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__restrict__ keys, uint32_t n,
-                                                              int shift, uint32_t *__restrict__ hist,
-                                                              uint32_t nblocks)
+__global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__restrict__ keys,
+                                                              const SortTile *__restrict__ tiles, int shift,
+                                                              uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t h[kRadixBuckets];
     const int t = threadIdx.x;
     for (int d = t; d < kRadixBuckets; d += kSortThreads) h[d] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * (uint32_t) kSortTile;
+    const SortTile tl = tiles[blockIdx.x];
 #pragma unroll 4
     for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t idx = base + (uint32_t) (k * kSortThreads + t);
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadixBuckets - 1)], 1u);
+        const uint32_t q = (uint32_t) (k * kSortThreads + t);
+        if (q < tl.count) atomicAdd(&h[((keys[tl.start + q] - tl.coff) >> shift) & (kRadixBuckets - 1)], 1u);
     }
     __syncthreads();
-    for (int d = t; d < kRadixBuckets; d += kSortThreads) hist[(uint32_t) d * nblocks + blockIdx.x] = h[d];
+    for (int d = t; d < kRadixBuckets; d += kSortThreads) hist[tl.hbase + (uint32_t) d * tl.stride] = h[d];
 }
 
 // Exclusive scan of a 4096-element tile held 4 per thread by a 1024-thread block:
@@ -318,7 +318,8 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data
 // offset among the equal digits before it, with two barriers per tile.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
-    uint32_t *__restrict__ vals_out, uint32_t n, int shift, const uint32_t *__restrict__ hist, uint32_t nblocks)
+    uint32_t *__restrict__ vals_out, const SortTile *__restrict__ tiles, int shift,
+    const uint32_t *__restrict__ hist)
 {
     constexpr int W = kSortThreads / 64;
     static_assert(kRadixBuckets == kSortThreads, "one thread scans one digit");
@@ -326,25 +327,25 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     __shared__ uint32_t base_s[kRadixBuckets];
     __shared__ uint16_t cnt[kSortItems * W][kRadixBuckets];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    base_s[t] = hist[(uint32_t) t * nblocks + blockIdx.x];
+    const SortTile tl = tiles[blockIdx.x];
+    base_s[t] = hist[tl.hbase + (uint32_t) t * tl.stride];
     {
         uint32_t *c32 = reinterpret_cast<uint32_t *>(&cnt[0][0]);
         for (int q = t; q < kSortItems * W * kRadixBuckets / 2; q += kSortThreads) c32[q] = 0u;
     }
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t tile = blockIdx.x * (uint32_t) kSortTile;
     uint32_t keys[kSortItems], vals[kSortItems], rank[kSortItems];
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t idx = tile + (uint32_t) (k * kSortThreads + t);
-        keys[k] = idx < n ? keys_in[idx] : 0u;
-        vals[k] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0u;
+        const uint32_t q = (uint32_t) (k * kSortThreads + t), idx = tl.start + q;
+        keys[k] = q < tl.count ? keys_in[idx] : 0u;
+        vals[k] = q < tl.count ? (vals_in ? vals_in[idx] : idx) : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const bool valid = tile + (uint32_t) (k * kSortThreads + t) < n;
-        const uint32_t dig = (keys[k] >> shift) & (kRadixBuckets - 1);
+        const bool valid = (uint32_t) (k * kSortThreads + t) < tl.count;
+        const uint32_t dig = ((keys[k] - tl.coff) >> shift) & (kRadixBuckets - 1);
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
@@ -368,8 +369,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        if (tile + (uint32_t) (k * kSortThreads + t) < n) {
-            const uint32_t dig = (keys[k] >> shift) & (kRadixBuckets - 1);
+        if ((uint32_t) (k * kSortThreads + t) < tl.count) {
+            const uint32_t dig = ((keys[k] - tl.coff) >> shift) & (kRadixBuckets - 1);
             const uint32_t pos = base_s[dig] + cnt[k * W + w][dig] + rank[k];
             keys_out[pos] = keys[k];
             vals_out[pos] = vals[k];
@@ -1418,13 +1419,13 @@ void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, doubl
 }
 
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out,
-                       uint32_t n, int shift, uint32_t *hist, uint32_t *partials, hipStream_t st)
+                       const SortTile *tiles, uint32_t ntiles, int shift, uint32_t *hist, uint32_t *partials,
+                       hipStream_t st)
 {
-    if (n == 0) return;
-    const uint32_t nb = sort_blocks(n);
-    const uint32_t m = nb * (uint32_t) kRadixBuckets;
+    if (ntiles == 0) return;
+    const uint32_t m = ntiles * (uint32_t) kRadixBuckets;
     const uint32_t sb = scan_blocks(m);
-    hipLaunchKernelGGL(k_radix_count, dim3(nb), dim3(kSortThreads), 0, st, keys_in, n, shift, hist, nb);
+    hipLaunchKernelGGL(k_radix_count, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, tiles, shift, hist);
     if (sb == 1) {
         hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, hist, m);
     } else {
@@ -1432,8 +1433,8 @@ void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_
         hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, partials, sb);
         hipLaunchKernelGGL(k_scan_apply, dim3(sb), dim3(1024), 0, st, hist, m, (const uint32_t *) partials);
     }
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, st, keys_in, vals_in, keys_out, vals_out,
-                       n, shift, (const uint32_t *) hist, nb);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in, keys_out, vals_out,
+                       tiles, shift, (const uint32_t *) hist);
 }
 
 void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells, uint32_t *start, hipStream_t st)
