@@ -85,6 +85,37 @@ def test_batch_grid_tables_equal_alone(hip):
             sim.close()
 
 
+def test_sort_passes_across_segment_sizes(hip):
+    """One scene of 41^3 = 68,921 cells (17-bit local keys: three 8-bit passes) with
+    two small scenes (one or two passes on their own): the batch sorts every
+    segment with the passes of the largest, by segment-local digits. Grid tables
+    and 8 steps equal each scene alone."""
+    lib = hip._lib
+    H = hip.scenes.H
+    edge = np.float32(41) * H
+    half = edge / np.float32(2)
+    big = dict(box=np.array([edge, edge, edge], np.float32),
+               pos=hip.scenes.block((-half + H, -half + H, -half + H), 16, 16, 16, seed=21))
+    big["n"] = len(big["pos"])
+    scs = [big] + small_scenes(hip, count=2, seed=23)
+    with make_batch(hip, scs) as b:
+        b.phase(lib.PHASE_GRID)
+        for i, sc in enumerate(scs):
+            sim = hip.HIPGridParticleSimulator(sc["box"], capacity=sc["n"])
+            sim.upload(sc["pos"])
+            sim.updateGrid()
+            for kind in (lib.DEBUG_PERM, lib.DEBUG_CELL, lib.DEBUG_CELL_START):
+                assert np.array_equal(b.debug(i, kind), sim.debug(kind)), "scene %d debug kind %d" % (i, kind)
+            sim.close()
+    with make_batch(hip, scs) as b:
+        b.step(8)
+        for i, sc in enumerate(scs):
+            p, v, _ = b.download(i)
+            pa, va, _ = run_alone(hip, sc, 8)
+            assert_bits_equal(p, pa, "scene %d positions" % i)
+            assert_bits_equal(v, va, "scene %d velocities" % i)
+
+
 def test_batch_vs_oracle(hip, orc):
     scs = small_scenes(hip, count=3, seed=21, nmin=500, nmax=1500)
     with make_batch(hip, scs) as b:
