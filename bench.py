@@ -182,7 +182,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=500, help="0 skips the batched measurement")
-    ap.add_argument("--per-scene-steps", type=int, default=20)
+    ap.add_argument("--per-scene-steps", type=int, default=60)
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="0 skips the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
