@@ -198,6 +198,33 @@ def test_masks_with_overflowing_runs(hip, orc):
     assert_bits_equal(v, o.state()["vel"], "crowded scene with masks (velocities)")
 
 
+def test_word_queues_with_many_words(hip, orc):
+    """A lattice at spacing h/4 (about 64 particles per cell): runs of ~190
+    candidates, six 32-bit mask words per row, more words per particle than any
+    queue variant holds, so the queues drain several times per wave. Every queue
+    variant equals the oracle bitwise."""
+    H = float(hip.scenes.H)
+    rng = np.random.default_rng(12)
+    g = np.arange(16, dtype=np.float64) * (H / 4) - 0.15
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = (pos + rng.uniform(-0.01 * H, 0.01 * H, pos.shape)).astype(np.float32)
+    o = orc.OracleSim(0.4, len(pos))
+    o.add_particles(pos)
+    o.set_order_mode(orc.ORDER_CANONICAL)
+    o.step(3)
+    want = o.state()
+    for v in range(hip._lib.NUM_VARIANTS):
+        if not hip.variant_shape(v)[4]:
+            continue
+        with hip.SceneBatch([(0.4, len(pos))]) as b:
+            b.upload(0, pos)
+            b.set_variant(v, hip._lib.NEIGHBOURS_MASKS)
+            b.step(3)
+            p, vel, _ = b.download(0)
+        assert_bits_equal(p, want["pos"], "queue variant %d positions" % v)
+        assert_bits_equal(vel, want["vel"], "queue variant %d velocities" % v)
+
+
 def test_autotune_changes_only_the_choice(hip):
     scs = small_scenes(hip, count=4, seed=9)
     with make_batch(hip, scs, max_neighbours=64) as a, make_batch(hip, scs, max_neighbours=64) as b:
