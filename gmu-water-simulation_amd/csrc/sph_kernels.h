@@ -76,7 +76,7 @@ struct NeighbourVariant {
     int stall;
     int queue;
 };
-constexpr int kNumVariants = 17;
+constexpr int kNumVariants = 21;
 constexpr int kDefaultVariant = 12;  // 64 threads, 8 candidates, a 16-word mask queue: fastest measured on the 1M dam break
 extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 

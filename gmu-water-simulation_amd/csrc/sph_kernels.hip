@@ -1322,17 +1322,20 @@ inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 // The compiled shapes of the two neighbour kernels, indexed by variant.
 template <int T, int U, int C, int kStall, bool kQueue>
 struct NeighbourKernels {
+    // the density pass of the word-queue variants keeps 64-thread workgroups
+    // (its fastest shape); `T` then shapes the force pass only
+    static constexpr int TD = kQueue ? 64 : T;
     template <bool kLists, bool kMasks>
     static void density_mode(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
                              const DensityFields &f, const NeighbourList &nl, const SceneTable &t, const Consts &k)
     {
-        const dim3 g(blocks_for(t.nslots, T));
+        const dim3 g(blocks_for(t.nslots, TD));
         if (f.ncount)
-            hipLaunchKernelGGL((k_density<T, U, kLists, kMasks, true>), g, dim3(T), 0, st, p, cell, start, f, nl,
-                               nl.masks, t, k);
+            hipLaunchKernelGGL((k_density<TD, U, kLists, kMasks, true>), g, dim3(TD), 0, st, p, cell, start, f,
+                               nl, nl.masks, t, k);
         else
-            hipLaunchKernelGGL((k_density<T, U, kLists, kMasks, false>), g, dim3(T), 0, st, p, cell, start, f, nl,
-                               nl.masks, t, k);
+            hipLaunchKernelGGL((k_density<TD, U, kLists, kMasks, false>), g, dim3(TD), 0, st, p, cell, start, f,
+                               nl, nl.masks, t, k);
     }
     static void density(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
                         const DensityFields &f, const NeighbourList &nl, const SceneTable &t, const Consts &k)
@@ -1388,7 +1391,11 @@ struct NeighbourKernels {
     X(13, 64, 8, 64, 0, true)   \
     X(14, 64, 4, 48, 0, true)   \
     X(15, 64, 8, 32, 0, true)   \
-    X(16, 64, 4, 32, 0, true)
+    X(16, 64, 4, 32, 0, true)   \
+    X(17, 128, 4, 32, 0, true)  \
+    X(18, 256, 4, 32, 0, true)  \
+    X(19, 256, 8, 48, 0, true)  \
+    X(20, 256, 4, 48, 0, true)
 
 }  // namespace
 

@@ -21,7 +21,7 @@ NEIGHBOUR_LISTS_OFF = 0xFFFFFFFF
  DEBUG_PRESSURE, DEBUG_ACCEL, DEBUG_IDS) = range(9)
 STAGES = ["hash", "sort", "cells", "reorder", "density", "forces", "collisions", "integrate"]
 EMIT_FOUNTAIN, EMIT_DISC = 0, 1
-NUM_VARIANTS = 17
+NUM_VARIANTS = 21
 DEFAULT_VARIANT = 12
 NEIGHBOURS_WALK, NEIGHBOURS_LISTS, NEIGHBOURS_MASKS = 0, 1, 2
 GROUP_SLOTS = 256

@@ -287,7 +287,7 @@ sph_status sph_set_graphs(sph_ctx *ctx, int enable);
  *                        kept (288 bytes per slot).
  * A new context uses variant 8 and SPH_NEIGHBOURS_MASKS, or SPH_NEIGHBOURS_LISTS when it
  * was created with max_neighbours > 0. Results are identical for every choice. */
-#define SPH_NUM_VARIANTS 17
+#define SPH_NUM_VARIANTS 21
 enum { SPH_NEIGHBOURS_WALK = 0, SPH_NEIGHBOURS_LISTS = 1, SPH_NEIGHBOURS_MASKS = 2 };
 sph_status sph_set_variant(sph_ctx *ctx, int variant, int mode);
 sph_status sph_get_variant(const sph_ctx *ctx, int *variant, int *mode);
