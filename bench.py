@@ -14,6 +14,9 @@ The same line carries the batched and served forms of the step:
     seed 3) advanced together for 500 steps (autotuned kernel shape, HIP graph
     replay), against the same scenes stepped by plain per-scene launches (one
     context and stream per scene, a bounded sample of steps);
+  * "fountain": a disc emitter adding 1,024 particles per step in the 1.6 m box
+    until 524,288, 2,000 steps with the on-device emitter, and its growth phase
+    against host-side generation plus a tail upload before every step;
   * "serving": Poisson arrivals of dam-break, fountain and cuboid-drop jobs
     (4,096-1,048,576 particles, 100-2,000 steps) into the native continuous-
     batching server for a bounded time; p50/p99 job latency and aggregate
@@ -158,6 +161,72 @@ def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
     return out
 
 
+def bench_fountain(sphhip, steps, host_steps):
+    """The fountain workload: a disc emitter at the floor centre of the 1.6 m box
+    adds 1,024 particles per step until 524,288 (step 512), run for `steps`
+    steps with the on-device emitter and graph replay. Beside it, the growth
+    phase with the particles generated on the host and appended before every
+    step (the reference's pattern, with a tail upload instead of its full
+    re-upload), for the first `host_steps` steps."""
+    from sphhip import _lib
+    f = sphhip.scenes.FOUNTAIN
+    box, rate, limit = f["box"], f["rate"], f["limit"]
+    b = sphhip.SceneBatch([(box, limit)])
+    b.set_emitter(0, kind=_lib.EMIT_DISC, rate=rate, limit=limit, centre=(0.0, 0.0), radius=f["radius"],
+                  speed=f["speed"], spread=f["spread"], seed=f["seed"])
+    b.set_graphs(True)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(host_steps)
+    b.sync()
+    grow = time.perf_counter() - t0
+    b.step(steps - host_steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    count = b.count(0)
+    b.close()
+    live = np.minimum(np.arange(1, steps + 1, dtype=np.int64) * rate, limit)
+    out = {"workload": "fountain: disc of radius 8h at the floor centre of the %.1f m box, %d particles per step with "
+                       "v_y = %.1f m/s +-%d%% (seed %d) until %d, %d steps" %
+                       (box, rate, f["speed"], int(f["spread"] * 100), f["seed"], limit, steps),
+           "steps": steps, "final_particles": count, "value": float(live.sum()) / wall,
+           "unit": "particle-steps/s", "ms_per_step": wall * 1e3 / steps, "graphs": True,
+           "growth": {"steps": host_steps, "device_emitter_ms_per_step": grow * 1e3 / host_steps}}
+
+    # Host-side injection: the same number of particles per step drawn on the
+    # host (uniform on the disc), appended before each step.
+    rng = np.random.default_rng(f["seed"])
+    floor_y = np.float32(-np.float32(box)) / np.float32(2.0)
+    sim = sphhip.HIPGridParticleSimulator(box, capacity=limit)
+
+    def batch_of(k):
+        r = f["radius"] * np.sqrt(rng.random(k))
+        a = 2 * np.pi * rng.random(k)
+        pos = np.stack([r * np.cos(a), np.full(k, floor_y), r * np.sin(a)], 1).astype(np.float32)
+        vel = np.zeros((k, 3), np.float32)
+        vel[:, 1] = f["speed"] * (1 + f["spread"] * (2 * rng.random(k) - 1))
+        return pos, vel
+
+    batches = [batch_of(rate) for _ in range(host_steps)]
+    sim.ensure_context(limit)
+    sim.sync()
+    t0 = time.perf_counter()
+    for i, (pos, vel) in enumerate(batches):
+        if i == 0:
+            sim.upload(pos, vel)
+        else:
+            sim.append(pos, vel)
+        sim.advance(1)
+    sim.sync()
+    host = time.perf_counter() - t0
+    sim.close()
+    out["growth"]["host_append_ms_per_step"] = host * 1e3 / host_steps
+    out["growth"]["how"] = ("steps 0-%d: device emitter under graph replay vs particles drawn on the host beforehand "
+                            "and appended (sph_append, %d particles) before each sph_step(ctx, 1)" %
+                            (host_steps - 1, rate))
+    return out
+
+
 def bench_serving(sphhip, seconds, rate, budget_gb, seed=4):
     budget = int(budget_gb * 2 ** 30)
     with sphhip.Server(budget, max_jobs=1024, chunk_steps=8) as srv:
@@ -183,6 +252,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=500, help="0 skips the batched measurement")
     ap.add_argument("--per-scene-steps", type=int, default=60)
+    ap.add_argument("--fountain-steps", type=int, default=2000, help="0 skips the fountain measurement")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="0 skips the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
@@ -265,6 +335,9 @@ def main():
     if args.batched_steps:
         print("bench: batched workload, %d steps" % args.batched_steps, file=sys.stderr, flush=True)
         line["batched"] = bench_batched(sphhip, args.batched_steps, args.per_scene_steps)
+    if args.fountain_steps:
+        print("bench: fountain workload, %d steps" % args.fountain_steps, file=sys.stderr, flush=True)
+        line["fountain"] = bench_fountain(sphhip, args.fountain_steps, min(512, args.fountain_steps))
     if args.serve_seconds:
         print("bench: serving for %.0f s" % args.serve_seconds, file=sys.stderr, flush=True)
         line["serving"] = bench_serving(sphhip, args.serve_seconds, args.serve_rate, args.serve_budget_gb)

@@ -1015,7 +1015,7 @@ sph_status create_ctx(uint32_t pool, uint32_t max_scenes, uint32_t flags, uint32
         return fail(SPH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     sph_status st = alloc_buffers(c, 2 * max_scenes + 1);
-    if (!st && c->nb_mode == SPH_NEIGHBOURS_MASKS) st = dalloc(c, &c->masks, (size_t) 9 * 8 * round_up(c->pool, 64));
+    if (!st && c->nb_mode == SPH_NEIGHBOURS_MASKS) st = dalloc(c, &c->masks, (size_t) 9 * sph::kMaskWords * round_up(c->pool, 64));
     if (!st) st = rebuild_layout(c);
     if (st) {
         std::string msg = g_last_error;
@@ -1119,10 +1119,10 @@ uint64_t sph_slot_bytes(uint32_t max_neighbours)
 {
     const uint64_t lists = max_neighbours == SPH_NEIGHBOUR_LISTS_OFF ? 0 : max_neighbours;
     // pos/vel ping-pong 48, ids 8, keys and values 20, rho 4, records 40,
-    // accelerations 12, counts 8, row masks 288 (the default mode without
+    // accelerations 12, counts 8, row masks (the default mode without
     // lists), list entries, and 4 for the histograms, the group map and the
     // scene's share of the cell table
-    return 48 + 8 + 20 + 4 + 4 * sph::kRecWords + 12 + 8 + (lists ? 4 * lists : 288) + 4;
+    return 48 + 8 + 20 + 4 + 4 * sph::kRecWords + 12 + 8 + (lists ? 4 * lists : 9 * 4 * sph::kMaskWords) + 4;
 }
 
 int sph_grid_res(float box)
@@ -1426,7 +1426,7 @@ sph_status ensure_masks(sph_ctx *c)
 {
     if (c->masks) return SPH_OK;
     SPH_HIP(hipStreamSynchronize(c->stream));
-    return dalloc(c, &c->masks, (size_t) 9 * 8 * round_up(c->pool, 64));
+    return dalloc(c, &c->masks, (size_t) 9 * sph::kMaskWords * round_up(c->pool, 64));
 }
 
 sph_status sph_set_variant(sph_ctx *c, int variant, int mode)

@@ -40,6 +40,15 @@ enum NeighbourMode : int { kNbWalk = 0, kNbLists = 1, kNbMasks = 2 };
 
 constexpr uint32_t kRecWords = 10;
 
+// Row masks: each of a particle's 9 rows holds up to kMaskBits candidates in
+// kMaskWords words of kMaskWordBits bits. 512 candidates cover the densest rows
+// measured (about 325 in a splash of the 1M dam break, 312 in the settled
+// fountain pool); a longer run makes the particle walk its cells in the force
+// pass, and the wave carrying it finishes last.
+constexpr uint32_t kMaskWordBits = 32;
+constexpr uint32_t kMaskWords = 16;
+constexpr uint32_t kMaskBits = kMaskWordBits * kMaskWords;
+
 struct NeighbourList {
     uint32_t *idx;
     uint32_t *cnt;

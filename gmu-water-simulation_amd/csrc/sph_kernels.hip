@@ -640,10 +640,6 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
 // has no masks (its force pass walks). Stored word-major per wave: word w of row
 // r of slot i is masks[(((i / 64) * 9 + r) * kMaskWords + w) * 64 + i % 64], so
 // the lanes of a wave write and read each word with one coalesced access.
-constexpr uint32_t kMaskWordBits = 32;
-constexpr uint32_t kMaskWords = 8;
-constexpr uint32_t kMaskBits = kMaskWordBits * kMaskWords;
-
 __device__ __forceinline__ size_t mask_word(uint32_t i, int r, uint32_t w)
 {
     return ((size_t) ((i >> 6) * 9u + (uint32_t) r) * kMaskWords + w) * 64u + (i & 63u);

@@ -284,8 +284,8 @@ sph_status sph_set_graphs(sph_ctx *ctx, int enable);
  *   SPH_NEIGHBOURS_LISTS reads neighbour lists the density pass kept (only if the context
  *                        was created with max_neighbours > 0);
  *   SPH_NEIGHBOURS_MASKS reads per-row masks of the accepted candidates the density pass
- *                        kept (288 bytes per slot).
- * A new context uses variant 8 and SPH_NEIGHBOURS_MASKS, or SPH_NEIGHBOURS_LISTS when it
+ *                        kept (576 bytes per slot).
+ * A new context uses variant 12 and SPH_NEIGHBOURS_MASKS, or SPH_NEIGHBOURS_LISTS when it
  * was created with max_neighbours > 0. Results are identical for every choice. */
 #define SPH_NUM_VARIANTS 21
 enum { SPH_NEIGHBOURS_WALK = 0, SPH_NEIGHBOURS_LISTS = 1, SPH_NEIGHBOURS_MASKS = 2 };

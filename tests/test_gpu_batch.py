@@ -181,8 +181,9 @@ def test_variants_and_modes_identical(hip):
 
 
 def test_masks_with_overflowing_runs(hip, orc):
-    """Row masks hold 256 candidates per row; a crowded scene whose runs are longer
-    walks those particles instead: still bitwise the oracle."""
+    """Row masks hold 512 candidates per row; a crowded scene (about 375 particles
+    per cell, runs of 750 and more) walks those particles instead: still bitwise
+    the oracle."""
     rng = np.random.default_rng(5)
     pos = (rng.random((3000, 3), dtype=np.float32) * np.float32(0.09) - np.float32(0.2)).astype(np.float32)
     with hip.SceneBatch([(0.4, len(pos))]) as b:
@@ -196,6 +197,30 @@ def test_masks_with_overflowing_runs(hip, orc):
     o.step(3)
     assert_bits_equal(p, o.state()["pos"], "crowded scene with masks")
     assert_bits_equal(v, o.state()["vel"], "crowded scene with masks (velocities)")
+
+
+def test_masks_with_long_runs(hip, orc):
+    """A lattice at spacing h/5 (about 125 particles per cell): runs of about
+    375 candidates, 12 mask words per row, held by the masks (no walk). Every
+    variant equals the oracle bitwise."""
+    H = float(hip.scenes.H)
+    rng = np.random.default_rng(21)
+    g = np.arange(15, dtype=np.float64) * (H / 5) - 0.15
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = (pos + rng.uniform(-0.01 * H, 0.01 * H, pos.shape)).astype(np.float32)
+    o = orc.OracleSim(0.4, len(pos))
+    o.add_particles(pos)
+    o.set_order_mode(orc.ORDER_CANONICAL)
+    o.step(2)
+    want = o.state()
+    for v in range(hip._lib.NUM_VARIANTS):
+        with hip.SceneBatch([(0.4, len(pos))]) as b:
+            b.upload(0, pos)
+            b.set_variant(v, hip._lib.NEIGHBOURS_MASKS)
+            b.step(2)
+            p, vel, _ = b.download(0)
+        assert_bits_equal(p, want["pos"], "variant %d positions" % v)
+        assert_bits_equal(vel, want["vel"], "variant %d velocities" % v)
 
 
 def test_word_queues_with_many_words(hip, orc):
