@@ -218,10 +218,28 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__
     for (int d = t; d < kRadixBuckets; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const SortTile tl = tiles[blockIdx.x];
-#pragma unroll 4
+    // The keys arrive nearly sorted (the previous step's order), so neighbouring
+    // lanes mostly share a digit: each run of equal digits in a wave adds its
+    // length with one atomic, from its first lane.
+    const uint32_t lane = (uint32_t) t & 63u;
+    uint32_t kv[kSortItems];   // all loads in flight together (the kernel is latency-bound)
+#pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t q = (uint32_t) (k * kSortThreads + t);
-        if (q < tl.count) atomicAdd(&h[((keys[tl.start + q] - tl.coff) >> shift) & (kRadixBuckets - 1)], 1u);
+        kv[k] = q < tl.count ? keys[tl.start + q] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t q = (uint32_t) (k * kSortThreads + t);
+        const bool valid = q < tl.count;
+        const uint32_t dig = valid ? ((kv[k] - tl.coff) >> shift) & (kRadixBuckets - 1) : kRadixBuckets;
+        const uint32_t prev = (uint32_t) __shfl_up((int) dig, 1, 64);
+        const bool head = lane == 0 || dig != prev;
+        const uint64_t heads = __ballot(head);
+        // the next run's first lane, or 64
+        const uint64_t after = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+        const uint32_t len = (after ? (uint32_t) __builtin_ctzll(after) : 64u) - lane;
+        if (head && valid) atomicAdd(&h[dig], len);
     }
     __syncthreads();
     for (int d = t; d < kRadixBuckets; d += kSortThreads) hist[tl.hbase + (uint32_t) d * tl.stride] = h[d];
@@ -419,18 +437,47 @@ __global__ void k_cell_start(const uint32_t *__restrict__ keys, uint32_t n, uint
     start[c] = lo;
 }
 
+// Each thread moves kReorderSlots slots (blockDim apart), their loads in flight
+// together.
+constexpr uint32_t kReorderSlots = 4;
+
 __global__ void k_reorder(ParticleSoA src, ParticleSoA dst, const uint32_t *__restrict__ perm, uint32_t n)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t j = perm[i];
-    dst.px[i] = src.px[j];
-    dst.py[i] = src.py[j];
-    dst.pz[i] = src.pz[j];
-    dst.vx[i] = src.vx[j];
-    dst.vy[i] = src.vy[j];
-    dst.vz[i] = src.vz[j];
-    dst.id[i] = src.id[j];
+    const uint32_t i0 = blockIdx.x * blockDim.x * kReorderSlots + threadIdx.x;
+    uint32_t j[kReorderSlots];
+#pragma unroll
+    for (uint32_t q = 0; q < kReorderSlots; ++q) {
+        const uint32_t i = i0 + q * blockDim.x;
+        j[q] = i < n ? perm[i] : 0u;
+    }
+    float px[kReorderSlots], py[kReorderSlots], pz[kReorderSlots], vx[kReorderSlots], vy[kReorderSlots],
+        vz[kReorderSlots];
+    uint32_t id[kReorderSlots];
+#pragma unroll
+    for (uint32_t q = 0; q < kReorderSlots; ++q) {
+        if (i0 + q * blockDim.x < n) {
+            px[q] = src.px[j[q]];
+            py[q] = src.py[j[q]];
+            pz[q] = src.pz[j[q]];
+            vx[q] = src.vx[j[q]];
+            vy[q] = src.vy[j[q]];
+            vz[q] = src.vz[j[q]];
+            id[q] = src.id[j[q]];
+        }
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < kReorderSlots; ++q) {
+        const uint32_t i = i0 + q * blockDim.x;
+        if (i < n) {
+            dst.px[i] = px[q];
+            dst.py[i] = py[q];
+            dst.pz[i] = pz[q];
+            dst.vx[i] = vx[q];
+            dst.vy[i] = vy[q];
+            dst.vz[i] = vz[q];
+            dst.id[i] = id[q];
+        }
+    }
 }
 
 // Particles first .. first+n-1 of the current order written by id into xyz
@@ -1450,7 +1497,7 @@ void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32
                     hipStream_t st)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256)), dim3(256), 0, st, src, dst, perm, n);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256 * kReorderSlots)), dim3(256), 0, st, src, dst, perm, n);
 }
 
 void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
