@@ -256,6 +256,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="0 skips the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--only", choices=("batched", "fountain", "serving"),
+                    help="run only this part and print its object (for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scales HBM counters)")
     args = ap.parse_args()
@@ -271,6 +273,17 @@ def main():
 
     import sphhip
     from sphhip import _lib
+
+    if args.only:
+        print("bench: only the %s part" % args.only, file=sys.stderr, flush=True)
+        if args.only == "batched":
+            part = bench_batched(sphhip, args.batched_steps, args.per_scene_steps)
+        elif args.only == "fountain":
+            part = bench_fountain(sphhip, args.fountain_steps, min(512, args.fountain_steps))
+        else:
+            part = bench_serving(sphhip, args.serve_seconds, args.serve_rate, args.serve_budget_gb)
+        print(json.dumps({args.only: part}), flush=True)
+        return
 
     pos, w = sphhip.scenes.workload(args.workload)
     n = len(pos)
