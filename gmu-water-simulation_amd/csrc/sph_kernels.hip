@@ -397,44 +397,34 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 }
 
 // start[c] = the first sorted slot whose key is >= c (lower bound), for every
-// c in [0, ncells]. The previous step's start[c] is the starting guess: particles
-// move little per step, so the bound is usually there or a few slots away; from
-// a wrong guess (any value, e.g. after a layout change) an exponential search
-// and a binary search still find it. The result does not depend on the guess.
-__global__ void k_cell_start(const uint32_t *__restrict__ keys, uint32_t n, uint32_t ncells,
-                             uint32_t *__restrict__ start)
+// c in [0, ncells]. Slot i (0 <= i <= n, slot n standing for the end) writes i
+// into every cell in (keys[i - 1], keys[i]] (from cell 0 for i = 0, up to ncells
+// for i = n), so each cell is written once and nothing is searched. A lane whose
+// range is longer than kFillShort cells (empty regions of a scene, the cells
+// after its last particle) has it filled by the whole wave, 64 cells a store.
+constexpr uint32_t kFillShort = 8;
+
+__global__ __launch_bounds__(256) void k_cell_start(const uint32_t *__restrict__ keys, uint32_t n,
+                                                    uint32_t ncells, uint32_t *__restrict__ start)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > ncells) return;
-    uint32_t h = start[c];
-    if (h > n) h = n;
-    uint32_t lo, hi;
-    if (h < n && keys[h] < c) {   // the bound is right of h: keys[a - 1] < c
-        uint32_t a = h + 1, b = a, step = 1;
-        while (b < n && keys[b] < c) {
-            a = b + 1;
-            b = a + step;
-            step <<= 1;
-        }
-        lo = a;
-        hi = b < n ? b : n;
-    } else if (h > 0 && keys[h - 1] >= c) {   // left of h: keys[b] >= c
-        uint32_t b = h - 1, a = b, step = 1;
-        while (a > 0 && keys[a - 1] >= c) {
-            b = a - 1;
-            a = b >= step ? b - step : 0;
-            step <<= 1;
-        }
-        lo = a;
-        hi = b;
-    } else {
-        lo = hi = h;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, lane = threadIdx.x & 63u;
+    uint32_t lo = 1, len = 0;   // cells [lo, lo + len) get i
+    if (i <= n) {
+        lo = i == 0 ? 0u : keys[i - 1] + 1u;
+        const uint32_t hi = min(i < n ? keys[i] : ncells, ncells);   // keys are cells <= ncells
+        len = lo <= hi ? hi - lo + 1u : 0u;
     }
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (keys[mid] < c) lo = mid + 1; else hi = mid;
+    if (len <= kFillShort) {
+        for (uint32_t q = 0; q < len; ++q) start[lo + q] = i;
     }
-    start[c] = lo;
+    uint64_t longs = __ballot(len > kFillShort);
+    while (longs) {
+        const uint32_t l = (uint32_t) __builtin_ctzll(longs);
+        longs &= longs - 1u;
+        const uint32_t llo = (uint32_t) __shfl((int) lo, (int) l, 64), llen = (uint32_t) __shfl((int) len, (int) l, 64);
+        const uint32_t li = (uint32_t) __shfl((int) i, (int) l, 64);
+        for (uint64_t q = lane; q < llen; q += 64u) start[llo + (uint32_t) q] = li;
+    }
 }
 
 // Each thread moves kReorderSlots slots (blockDim apart), their loads in flight
@@ -1489,7 +1479,7 @@ void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_
 
 void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells, uint32_t *start, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(ncells + 1, 256)), dim3(256), 0, st, sorted_keys, n, ncells,
+    hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(n + 1, 256)), dim3(256), 0, st, sorted_keys, n, ncells,
                        start);
 }
 
