@@ -284,7 +284,7 @@ __device__ __forceinline__ uint32_t load4(const uint32_t *__restrict__ data, uin
     return v[0] + v[1] + v[2] + v[3];
 }
 
-// Single-block in-place exclusive scan of m values (used on the block totals).
+// Single-block in-place exclusive scan of m values (histograms of one scan tile).
 __global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t *__restrict__ data, uint32_t m)
 {
     __shared__ uint32_t wsum[17];
@@ -313,14 +313,18 @@ __global__ __launch_bounds__(1024) void k_scan_reduce(const uint32_t *__restrict
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-// Multi-block scan, step 3: exclusive scan of every tile plus its scanned total.
+// Multi-block scan, step 2: exclusive scan of every tile, offset by the totals of
+// the tiles before it (each block sums those itself: a few thousand at most).
 __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data, uint32_t m,
                                                      const uint32_t *__restrict__ partials)
 {
     __shared__ uint32_t wsum[17];
+    uint32_t before = 0, carry;
+    for (uint32_t q = threadIdx.x; q < blockIdx.x; q += 1024u) before += partials[q];
+    block_exclusive_scan(before, wsum, carry);
     const uint32_t idx = blockIdx.x * (uint32_t) kScanTile + (uint32_t) threadIdx.x * 4u;
     uint32_t v[4], total;
-    uint32_t run = partials[blockIdx.x] + block_exclusive_scan(load4(data, idx, m, v), wsum, total);
+    uint32_t run = carry + block_exclusive_scan(load4(data, idx, m, v), wsum, total);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (idx + q < m) data[idx + q] = run;
@@ -1470,7 +1474,6 @@ void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_
         hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, hist, m);
     } else {
         hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(1024), 0, st, (const uint32_t *) hist, m, partials);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, partials, sb);
         hipLaunchKernelGGL(k_scan_apply, dim3(sb), dim3(1024), 0, st, hist, m, (const uint32_t *) partials);
     }
     hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in, keys_out, vals_out,
