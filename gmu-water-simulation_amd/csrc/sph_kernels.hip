@@ -333,30 +333,29 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data
 }
 
 // Stable scatter of one LSD radix pass. The tile's items (slot order: round k
-// covers slots tile + k*256 .. +255) are all loaded up front; equal digits
-// inside a wave are matched with 8 ballots and ranked by popcount of the lower
-// lanes. Each (round, wave) records its digit counts in LDS; one exclusive scan
-// per digit over the tile's (round, wave) sequence then gives every item its
-// offset among the equal digits before it, with two barriers per tile.
+// covers slots tile + k*256 .. +255) are all loaded up front, then placed round
+// by round: equal digits inside a wave are matched with 8 ballots and ranked by
+// popcount of the lower lanes; the first lane of each digit records the wave's
+// count in LDS; the thread of each digit turns the round's four wave counts into
+// absolute offsets (its running base, from the pass histogram, in a register);
+// every item stores at offset + rank. Two barriers a round; the count table is
+// double-buffered (the thread of a digit clears the other buffer's column), so
+// the kernel needs 8 KB of LDS.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, const SortTile *__restrict__ tiles, int shift,
     const uint32_t *__restrict__ hist)
 {
     constexpr int W = kSortThreads / 64;
-    static_assert(kRadixBuckets == kSortThreads, "one thread scans one digit");
-    static_assert(kSortTile <= 65535, "tile offsets are 16-bit");
-    __shared__ uint32_t base_s[kRadixBuckets];
-    __shared__ uint16_t cnt[kSortItems * W][kRadixBuckets];
+    static_assert(kRadixBuckets == kSortThreads, "one thread per digit");
+    __shared__ uint32_t off[2][W][kRadixBuckets];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const SortTile tl = tiles[blockIdx.x];
-    base_s[t] = hist[tl.hbase + (uint32_t) t * tl.stride];
-    {
-        uint32_t *c32 = reinterpret_cast<uint32_t *>(&cnt[0][0]);
-        for (int q = t; q < kSortItems * W * kRadixBuckets / 2; q += kSortThreads) c32[q] = 0u;
-    }
+    uint32_t base = hist[tl.hbase + (uint32_t) t * tl.stride];
+#pragma unroll
+    for (int q = 0; q < W; ++q) off[0][q][t] = 0u;
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    uint32_t keys[kSortItems], vals[kSortItems], rank[kSortItems];
+    uint32_t keys[kSortItems], vals[kSortItems];
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t q = (uint32_t) (k * kSortThreads + t), idx = tl.start + q;
@@ -366,6 +365,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
+        const int bf = k & 1;
         const bool valid = (uint32_t) (k * kSortThreads + t) < tl.count;
         const uint32_t dig = ((keys[k] - tl.coff) >> shift) & (kRadixBuckets - 1);
         uint64_t m = __ballot(valid);
@@ -375,25 +375,24 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
-        rank[k] = (uint32_t) __popcll(m & below);
-        if (valid && rank[k] == 0) cnt[k * W + w][dig] = (uint16_t) __popcll(m);
-    }
-    __syncthreads();
-    {
-        uint32_t run = 0;
-#pragma unroll 16
-        for (int q = 0; q < kSortItems * W; ++q) {
-            const uint32_t v = cnt[q][t];
-            cnt[q][t] = (uint16_t) run;
-            run += v;
-        }
-    }
-    __syncthreads();
+        const uint32_t rank = (uint32_t) __popcll(m & below);
+        if (valid && rank == 0) off[bf][w][dig] = (uint32_t) __popcll(m);
+        __syncthreads();
+        {
+            // the thread of digit t: wave counts -> absolute offsets; clear the other buffer
+            uint32_t run = base;
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        if ((uint32_t) (k * kSortThreads + t) < tl.count) {
-            const uint32_t dig = ((keys[k] - tl.coff) >> shift) & (kRadixBuckets - 1);
-            const uint32_t pos = base_s[dig] + cnt[k * W + w][dig] + rank[k];
+            for (int q = 0; q < W; ++q) {
+                const uint32_t c = off[bf][q][t];
+                off[bf][q][t] = run;
+                run += c;
+                off[bf ^ 1][q][t] = 0u;
+            }
+            base = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = off[bf][w][dig] + rank;
             keys_out[pos] = keys[k];
             vals_out[pos] = vals[k];
         }
