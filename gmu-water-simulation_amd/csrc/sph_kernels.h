@@ -86,7 +86,7 @@ struct NeighbourVariant {
     int queue;
 };
 constexpr int kNumVariants = 21;
-constexpr int kDefaultVariant = 12;  // 64 threads, 8 candidates, a 16-word mask queue: fastest measured on the 1M dam break
+constexpr int kDefaultVariant = 12;  // 64 threads, 8 candidates, a 24-entry word queue: fastest measured on the 1M dam break
 extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 
 // Emitters (one workgroup each) add particles at the start of a step and update
