@@ -317,6 +317,16 @@ def main():
     dom_ms = stages[dominant]
     achieved = ALG_BYTES[dominant] * n / (dom_ms * 1e-3) / 1e9
     step_bytes = sum(ALG_BYTES.values()) * n + ALG_BYTES_PER_CELL * ncells
+    # the streaming stages against the HBM peak (algorithmic bytes / HIP-event stage time); the
+    # cell table reads the sorted keys and writes one start per cell
+    stream_bytes = {"hash": ALG_BYTES["hash"] * n, "sort": ALG_BYTES["sort"] * n,
+                    "cells": 4 * n + 4 * (ncells + 1), "reorder": ALG_BYTES["reorder"] * n}
+    stage_roofline = {}
+    for name, nbytes in stream_bytes.items():
+        ms = stages.get(name, 0.0)
+        if ms > 0:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            stage_roofline[name] = {"ms": ms, "alg_bytes": nbytes, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
     traffic = load_traffic(args.workload)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
@@ -343,6 +353,7 @@ def main():
         "device": sim.getSelectedDevice(),
         "event_ms_per_step": ev_ms / args.steps,
         "stage_ms": stages,
+        "stage_roofline": stage_roofline,
         "roofline": roofline,
     }
     sim.close()
