@@ -230,8 +230,15 @@ def bench_fountain(sphhip, steps, host_steps):
 
 def bench_serving(sphhip, seconds, rate, budget_gb, seed=4):
     budget = int(budget_gb * 2 ** 30)
+    submits = [0]
+
+    def log(msg):   # every 20th submission, so that a long run reports its progress
+        submits[0] += 1
+        if submits[0] % 20 == 0:
+            print("bench: serving %s" % msg, file=sys.stderr, flush=True)
+
     with sphhip.Server(budget, max_jobs=1024, chunk_steps=8) as srv:
-        recs, st = sphhip.serving.poisson_run(srv, rate, seconds, seed=seed)
+        recs, st = sphhip.serving.poisson_run(srv, rate, seconds, seed=seed, log=log)
     lat = np.array([r["latency"] for r in recs]) if recs else np.zeros(1)
     return {"workload": "Poisson arrivals at %.2f jobs/s for %.0f s (seed %d): dam break, fountain or cuboid drop, "
                         "4,096-1,048,576 particles (log-uniform), 100-2,000 steps (uniform)" % (rate, seconds, seed),
