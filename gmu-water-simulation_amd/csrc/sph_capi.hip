@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -212,6 +214,29 @@ Consts make_consts()
 sph_status report(sph_status code, const std::string &msg)
 {
     return fail(code, msg);
+}
+
+// The status of the C++ exception being handled (called in a catch block of an
+// entry point, which never lets one cross the C ABI).
+sph_status host_exception() noexcept
+{
+    sph_status code = SPH_ERR_INVALID;
+    const char *msg = "host error";
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        code = SPH_ERR_OOM;
+        msg = "host memory allocation failed";
+    } catch (const std::exception &e) {
+        msg = e.what();
+    } catch (...) {
+    }
+    try {
+        g_last_error = msg;
+    } catch (...) {
+        g_last_error.clear();
+    }
+    return code;
 }
 
 }  // namespace sph
@@ -1133,292 +1158,392 @@ int sph_grid_res(float box)
 
 sph_status sph_create(const sph_params *params, sph_ctx **out)
 {
-    if (!params || !out) return fail(SPH_ERR_INVALID, "sph_create: null argument");
-    sph_scene_desc d;
-    std::memcpy(d.box, params->box, sizeof(d.box));
-    std::memcpy(d.gravity, params->gravity, sizeof(d.gravity));
-    d.dt = params->dt;
-    d.capacity = params->capacity;
-    if (params->capacity == 0) return fail(SPH_ERR_INVALID, "sph_create: capacity must be > 0");
-    return create_ctx(params->capacity, 1, params->flags, params->max_neighbours, &d, 1, out, "sph_create");
+    try {
+        if (!params || !out) return fail(SPH_ERR_INVALID, "sph_create: null argument");
+        sph_scene_desc d;
+        std::memcpy(d.box, params->box, sizeof(d.box));
+        std::memcpy(d.gravity, params->gravity, sizeof(d.gravity));
+        d.dt = params->dt;
+        d.capacity = params->capacity;
+        if (params->capacity == 0) return fail(SPH_ERR_INVALID, "sph_create: capacity must be > 0");
+        return create_ctx(params->capacity, 1, params->flags, params->max_neighbours, &d, 1, out, "sph_create");
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_batch_create(const sph_batch_params *bp, const sph_scene_desc *scenes, uint32_t nscenes,
                             sph_ctx **out)
 {
-    if (!bp) return fail(SPH_ERR_INVALID, "sph_batch_create: null parameters");
-    const uint32_t max_scenes = bp->max_scenes ? bp->max_scenes : nscenes;
-    return create_ctx(bp->pool, max_scenes, bp->flags, bp->max_neighbours, scenes, nscenes, out, "sph_batch_create");
+    try {
+        if (!bp) return fail(SPH_ERR_INVALID, "sph_batch_create: null parameters");
+        const uint32_t max_scenes = bp->max_scenes ? bp->max_scenes : nscenes;
+        return create_ctx(bp->pool, max_scenes, bp->flags, bp->max_neighbours, scenes, nscenes, out, "sph_batch_create");
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 void sph_destroy(sph_ctx *ctx)
 {
-    release(ctx);
+    try {
+        release(ctx);
+    } catch (...) {
+        (void) sph::host_exception();
+    }
 }
 
 sph_status sph_scene_add(sph_ctx *c, const sph_scene_desc *desc, uint32_t *scene)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (!desc || !scene) return fail(SPH_ERR_INVALID, "sph_scene_add: null argument");
-    SPH_TRY(between_steps(c, "sph_scene_add"));
-    sph::Scene s;
-    SPH_TRY(init_scene(desc, s, "sph_scene_add"));
-    uint32_t id = 0;
-    while (id < c->scenes.size() && c->scenes[id].used) ++id;
-    if (id == c->scenes.size()) return fail(SPH_ERR_CAPACITY, "sph_scene_add: all scene ids are in use");
-    SPH_TRY(sync_counts(c));
-    uint32_t poff = 0, seg = 0;
-    if (!find_range(c, desc->capacity, poff, seg))
-        return fail(SPH_ERR_CAPACITY, "sph_scene_add: no free range of " + std::to_string(desc->capacity) +
-                                          " slots in the pool");
-    s.poff = poff;
-    s.cap = seg;
-    SceneRec &r = c->scenes[id];
-    r = SceneRec{};
-    r.used = true;
-    r.req_cap = desc->capacity;
-    r.s = s;
-    touch_layout(c);
-    *scene = id;
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (!desc || !scene) return fail(SPH_ERR_INVALID, "sph_scene_add: null argument");
+        SPH_TRY(between_steps(c, "sph_scene_add"));
+        sph::Scene s;
+        SPH_TRY(init_scene(desc, s, "sph_scene_add"));
+        uint32_t id = 0;
+        while (id < c->scenes.size() && c->scenes[id].used) ++id;
+        if (id == c->scenes.size()) return fail(SPH_ERR_CAPACITY, "sph_scene_add: all scene ids are in use");
+        SPH_TRY(sync_counts(c));
+        uint32_t poff = 0, seg = 0;
+        if (!find_range(c, desc->capacity, poff, seg))
+            return fail(SPH_ERR_CAPACITY, "sph_scene_add: no free range of " + std::to_string(desc->capacity) +
+                                              " slots in the pool");
+        s.poff = poff;
+        s.cap = seg;
+        SceneRec &r = c->scenes[id];
+        r = SceneRec{};
+        r.used = true;
+        r.req_cap = desc->capacity;
+        r.s = s;
+        touch_layout(c);
+        *scene = id;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_remove(sph_ctx *c, uint32_t scene)
 {
-    SPH_TRY(check_scene(c, scene, "sph_scene_remove"));
-    SPH_TRY(between_steps(c, "sph_scene_remove"));
-    SPH_TRY(sync_counts(c));
-    c->scenes[scene] = SceneRec{};
-    touch_layout(c);
-    return SPH_OK;
+    try {
+        SPH_TRY(check_scene(c, scene, "sph_scene_remove"));
+        SPH_TRY(between_steps(c, "sph_scene_remove"));
+        SPH_TRY(sync_counts(c));
+        c->scenes[scene] = SceneRec{};
+        touch_layout(c);
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_upload(sph_ctx *c, const float *pos3, const float *vel3, const uint32_t *ids, uint32_t n)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    SPH_TRY(scene_write(c, 0, pos3, vel3, ids, n, false, "sph_upload"));
-    c->next_phase = SPH_PHASE_GRID;
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        SPH_TRY(scene_write(c, 0, pos3, vel3, ids, n, false, "sph_upload"));
+        c->next_phase = SPH_PHASE_GRID;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_append(sph_ctx *c, const float *pos3, const float *vel3, const uint32_t *ids, uint32_t n)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (n == 0) return SPH_OK;
-    SPH_TRY(between_steps(c, "sph_append"));
-    return scene_write(c, 0, pos3, vel3, ids, n, true, "sph_append");
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (n == 0) return SPH_OK;
+        SPH_TRY(between_steps(c, "sph_append"));
+        return scene_write(c, 0, pos3, vel3, ids, n, true, "sph_append");
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_upload(sph_ctx *c, uint32_t scene, const float *pos3, const float *vel3, const uint32_t *ids,
                             uint32_t n)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    SPH_TRY(between_steps(c, "sph_scene_upload"));
-    return scene_write(c, scene, pos3, vel3, ids, n, false, "sph_scene_upload");
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        SPH_TRY(between_steps(c, "sph_scene_upload"));
+        return scene_write(c, scene, pos3, vel3, ids, n, false, "sph_scene_upload");
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_append(sph_ctx *c, uint32_t scene, const float *pos3, const float *vel3, const uint32_t *ids,
                             uint32_t n)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (n == 0) return SPH_OK;
-    SPH_TRY(between_steps(c, "sph_scene_append"));
-    return scene_write(c, scene, pos3, vel3, ids, n, true, "sph_scene_append");
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (n == 0) return SPH_OK;
+        SPH_TRY(between_steps(c, "sph_scene_append"));
+        return scene_write(c, scene, pos3, vel3, ids, n, true, "sph_scene_append");
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_set_gravity(sph_ctx *c, uint32_t scene, float gx, float gy, float gz)
 {
-    SPH_TRY(check_scene(c, scene, "sph_set_gravity"));
-    SPH_TRY(sync_counts(c));
-    sph::Scene &s = c->scenes[scene].s;
-    s.gx = gx;
-    s.gy = gy;
-    s.gz = gz;
-    // only the gravity of one table entry changes: patch it in place so that the
-    // layout, the keys and captured graphs stay valid
-    if (!c->layout_dirty) {
-        for (size_t t = 0; t < c->table_sid.size(); ++t)
-            if (c->table_sid[t] == scene) {
-                c->table_h[t].gx = gx;
-                c->table_h[t].gy = gy;
-                c->table_h[t].gz = gz;
-                SPH_HIP(hipStreamSynchronize(c->stream));
-                SPH_HIP(hipMemcpy(&c->table_d[t].gx, &c->table_h[t].gx, 3 * sizeof(float), hipMemcpyHostToDevice));
-            }
+    try {
+        SPH_TRY(check_scene(c, scene, "sph_set_gravity"));
+        SPH_TRY(sync_counts(c));
+        sph::Scene &s = c->scenes[scene].s;
+        s.gx = gx;
+        s.gy = gy;
+        s.gz = gz;
+        // only the gravity of one table entry changes: patch it in place so that the
+        // layout, the keys and captured graphs stay valid
+        if (!c->layout_dirty) {
+            for (size_t t = 0; t < c->table_sid.size(); ++t)
+                if (c->table_sid[t] == scene) {
+                    c->table_h[t].gx = gx;
+                    c->table_h[t].gy = gy;
+                    c->table_h[t].gz = gz;
+                    SPH_HIP(hipStreamSynchronize(c->stream));
+                    SPH_HIP(hipMemcpy(&c->table_d[t].gx, &c->table_h[t].gx, 3 * sizeof(float), hipMemcpyHostToDevice));
+                }
+        }
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    return SPH_OK;
 }
 
 sph_status sph_set_gravity(sph_ctx *c, float gx, float gy, float gz)
 {
-    return sph_scene_set_gravity(c, 0, gx, gy, gz);
+    try {
+        return sph_scene_set_gravity(c, 0, gx, gy, gz);
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_set_emitter(sph_ctx *c, uint32_t scene, const sph_emitter *e)
 {
-    SPH_TRY(check_scene(c, scene, "sph_scene_set_emitter"));
-    SPH_TRY(between_steps(c, "sph_scene_set_emitter"));
-    SPH_TRY(sync_counts(c));
-    SceneRec &r = c->scenes[scene];
-    if (!e) {
-        r.has_emitter = false;
+    try {
+        SPH_TRY(check_scene(c, scene, "sph_scene_set_emitter"));
+        SPH_TRY(between_steps(c, "sph_scene_set_emitter"));
+        SPH_TRY(sync_counts(c));
+        SceneRec &r = c->scenes[scene];
+        if (!e) {
+            r.has_emitter = false;
+            touch_layout(c);
+            return SPH_OK;
+        }
+        if (e->kind != SPH_EMIT_FOUNTAIN && e->kind != SPH_EMIT_DISC)
+            return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: unknown kind " + std::to_string(e->kind));
+        if (e->kind == SPH_EMIT_DISC) {
+            if (e->rate == 0) return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: rate must be > 0");
+            if (!(e->radius >= 0.0f) || !std::isfinite(e->radius) || !std::isfinite(e->speed) ||
+                !(e->spread >= 0.0f) || !std::isfinite(e->spread) || !std::isfinite(e->centre[0]) ||
+                !std::isfinite(e->centre[1]))
+                return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: disc parameters must be finite, radius >= 0");
+        }
+        sph::Emitter &m = r.em;
+        m = sph::Emitter{};
+        m.kind = e->kind;
+        m.rate = e->rate;
+        // never beyond the capacity the caller reserved
+        m.limit = e->kind == SPH_EMIT_DISC ? std::min(e->limit, r.req_cap) : e->limit;
+        if (e->kind == SPH_EMIT_FOUNTAIN && (e->limit > r.req_cap))
+            return fail(SPH_ERR_CAPACITY, "sph_scene_set_emitter: fountain limit above the scene's capacity");
+        m.cx = e->centre[0];
+        m.cz = e->centre[1];
+        m.radius = e->radius;
+        m.speed = e->speed;
+        m.spread = e->spread;
+        m.seed_lo = (uint32_t) e->seed;
+        m.seed_hi = (uint32_t) (e->seed >> 32);
+        r.has_emitter = true;
         touch_layout(c);
         return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    if (e->kind != SPH_EMIT_FOUNTAIN && e->kind != SPH_EMIT_DISC)
-        return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: unknown kind " + std::to_string(e->kind));
-    if (e->kind == SPH_EMIT_DISC) {
-        if (e->rate == 0) return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: rate must be > 0");
-        if (!(e->radius >= 0.0f) || !std::isfinite(e->radius) || !std::isfinite(e->speed) ||
-            !(e->spread >= 0.0f) || !std::isfinite(e->spread) || !std::isfinite(e->centre[0]) ||
-            !std::isfinite(e->centre[1]))
-            return fail(SPH_ERR_INVALID, "sph_scene_set_emitter: disc parameters must be finite, radius >= 0");
-    }
-    sph::Emitter &m = r.em;
-    m = sph::Emitter{};
-    m.kind = e->kind;
-    m.rate = e->rate;
-    // never beyond the capacity the caller reserved
-    m.limit = e->kind == SPH_EMIT_DISC ? std::min(e->limit, r.req_cap) : e->limit;
-    if (e->kind == SPH_EMIT_FOUNTAIN && (e->limit > r.req_cap))
-        return fail(SPH_ERR_CAPACITY, "sph_scene_set_emitter: fountain limit above the scene's capacity");
-    m.cx = e->centre[0];
-    m.cz = e->centre[1];
-    m.radius = e->radius;
-    m.speed = e->speed;
-    m.spread = e->spread;
-    m.seed_lo = (uint32_t) e->seed;
-    m.seed_hi = (uint32_t) (e->seed >> 32);
-    r.has_emitter = true;
-    touch_layout(c);
-    return SPH_OK;
 }
 
 sph_status sph_phase(sph_ctx *c, int phase, float *ms_out)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (ms_out) SPH_HIP(hipEventRecord(c->span[0], c->stream));
-    SPH_TRY(run_phase(c, phase));
-    if (ms_out) {
-        SPH_HIP(hipEventRecord(c->span[1], c->stream));
-        SPH_HIP(hipEventSynchronize(c->span[1]));
-        SPH_HIP(hipEventElapsedTime(ms_out, c->span[0], c->span[1]));
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (ms_out) SPH_HIP(hipEventRecord(c->span[0], c->stream));
+        SPH_TRY(run_phase(c, phase));
+        if (ms_out) {
+            SPH_HIP(hipEventRecord(c->span[1], c->stream));
+            SPH_HIP(hipEventSynchronize(c->span[1]));
+            SPH_HIP(hipEventElapsedTime(ms_out, c->span[0], c->span[1]));
+        }
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    return SPH_OK;
 }
 
 sph_status sph_step(sph_ctx *c, uint32_t nsteps, float *ms_out)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (c->next_phase != SPH_PHASE_GRID) return fail(SPH_ERR_STATE, "sph_step: a phase sequence is in progress");
-    SPH_TRY(check_runnable(c, "sph_step"));
-    if (ms_out) SPH_HIP(hipEventRecord(c->span[0], c->stream));
-    for (uint32_t s = 0; s < nsteps; ++s) {
-        SPH_TRY(check_fault(c));
-        SPH_TRY(run_step(c));
-        c->steps_done++;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (c->next_phase != SPH_PHASE_GRID) return fail(SPH_ERR_STATE, "sph_step: a phase sequence is in progress");
+        SPH_TRY(check_runnable(c, "sph_step"));
+        if (ms_out) SPH_HIP(hipEventRecord(c->span[0], c->stream));
+        for (uint32_t s = 0; s < nsteps; ++s) {
+            SPH_TRY(check_fault(c));
+            SPH_TRY(run_step(c));
+            c->steps_done++;
+        }
+        if (ms_out) {
+            SPH_HIP(hipEventRecord(c->span[1], c->stream));
+            SPH_HIP(hipEventSynchronize(c->span[1]));
+            SPH_HIP(hipEventElapsedTime(ms_out, c->span[0], c->span[1]));
+        }
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    if (ms_out) {
-        SPH_HIP(hipEventRecord(c->span[1], c->stream));
-        SPH_HIP(hipEventSynchronize(c->span[1]));
-        SPH_HIP(hipEventElapsedTime(ms_out, c->span[0], c->span[1]));
-    }
-    return SPH_OK;
 }
 
 sph_status sph_download(sph_ctx *c, float *pos3, float *vel3, uint32_t *ids, uint32_t n, int by_id)
 {
-    return scene_download(c, 0, pos3, vel3, ids, n, by_id, "sph_download");
+    try {
+        return scene_download(c, 0, pos3, vel3, ids, n, by_id, "sph_download");
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_download(sph_ctx *c, uint32_t scene, float *pos3, float *vel3, uint32_t *ids, uint32_t n,
                               int by_id)
 {
-    return scene_download(c, scene, pos3, vel3, ids, n, by_id, "sph_scene_download");
+    try {
+        return scene_download(c, scene, pos3, vel3, ids, n, by_id, "sph_scene_download");
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_debug(sph_ctx *c, int kind, void *out, size_t bytes)
 {
-    return scene_debug(c, 0, kind, out, bytes);
+    try {
+        return scene_debug(c, 0, kind, out, bytes);
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_debug(sph_ctx *c, uint32_t scene, int kind, void *out, size_t bytes)
 {
-    return scene_debug(c, scene, kind, out, bytes);
+    try {
+        return scene_debug(c, scene, kind, out, bytes);
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_count(const sph_ctx *cc, uint32_t *n)
 {
-    return sph_scene_count(const_cast<sph_ctx *>(cc), 0, n);
+    try {
+        return sph_scene_count(const_cast<sph_ctx *>(cc), 0, n);
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_count(sph_ctx *c, uint32_t scene, uint32_t *n)
 {
-    SPH_TRY(check_scene(c, scene, "sph_count"));
-    if (!n) return fail(SPH_ERR_INVALID, "sph_count: null output");
-    SPH_TRY(sync_counts(c));
-    *n = c->scenes[scene].s.n;
-    return SPH_OK;
+    try {
+        SPH_TRY(check_scene(c, scene, "sph_count"));
+        if (!n) return fail(SPH_ERR_INVALID, "sph_count: null output");
+        SPH_TRY(sync_counts(c));
+        *n = c->scenes[scene].s.n;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_grid_dims(const sph_ctx *c, int *xyz, uint32_t *ncells)
 {
-    return sph_scene_grid(const_cast<sph_ctx *>(c), 0, xyz, ncells, nullptr, nullptr);
+    try {
+        return sph_scene_grid(const_cast<sph_ctx *>(c), 0, xyz, ncells, nullptr, nullptr);
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_scene_grid(sph_ctx *c, uint32_t scene, int *xyz, uint32_t *ncells, uint32_t *first_slot,
                           uint32_t *slots)
 {
-    SPH_TRY(check_scene(c, scene, "sph_grid_dims"));
-    const sph::Scene &s = c->scenes[scene].s;
-    if (xyz) {
-        xyz[0] = s.rx;
-        xyz[1] = s.ry;
-        xyz[2] = s.rz;
+    try {
+        SPH_TRY(check_scene(c, scene, "sph_grid_dims"));
+        const sph::Scene &s = c->scenes[scene].s;
+        if (xyz) {
+            xyz[0] = s.rx;
+            xyz[1] = s.ry;
+            xyz[2] = s.rz;
+        }
+        if (ncells) *ncells = s.ncells;
+        if (first_slot) *first_slot = s.poff;
+        if (slots) *slots = s.cap;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    if (ncells) *ncells = s.ncells;
-    if (first_slot) *first_slot = s.poff;
-    if (slots) *slots = s.cap;
-    return SPH_OK;
 }
 
 sph_status sph_batch_info(sph_ctx *c, uint32_t *scenes, uint32_t *pool, uint32_t *free_slots, uint32_t *active_slots)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    SPH_TRY(ensure_layout(c));
-    uint32_t used = 0;
-    uint64_t taken = 0;
-    for (const SceneRec &r : c->scenes)
-        if (r.used) {
-            ++used;
-            taken += r.s.cap;
-        }
-    if (scenes) *scenes = used;
-    if (pool) *pool = c->pool;
-    if (free_slots) *free_slots = (uint32_t) (c->pool - taken);
-    if (active_slots) *active_slots = c->nslots;
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        SPH_TRY(ensure_layout(c));
+        uint32_t used = 0;
+        uint64_t taken = 0;
+        for (const SceneRec &r : c->scenes)
+            if (r.used) {
+                ++used;
+                taken += r.s.cap;
+            }
+        if (scenes) *scenes = used;
+        if (pool) *pool = c->pool;
+        if (free_slots) *free_slots = (uint32_t) (c->pool - taken);
+        if (active_slots) *active_slots = c->nslots;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_set_graphs(sph_ctx *c, int enable)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    SPH_HIP(hipStreamSynchronize(c->stream));
-    drop_graphs(c);
-    c->use_graphs = enable != 0;
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+        c->use_graphs = enable != 0;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_variant_shape(int variant, int *threads, int *unroll, int *pairs, int *stall, int *queue)
 {
-    if (variant < 0 || variant >= sph::kNumVariants)
-        return fail(SPH_ERR_INVALID, "sph_variant_shape: no variant " + std::to_string(variant));
-    if (threads) *threads = sph::kNeighbourVariants[variant].threads;
-    if (unroll) *unroll = sph::kNeighbourVariants[variant].unroll;
-    if (pairs) *pairs = sph::kNeighbourVariants[variant].pairs;
-    if (stall) *stall = sph::kNeighbourVariants[variant].stall;
-    if (queue) *queue = sph::kNeighbourVariants[variant].queue;
-    return SPH_OK;
+    try {
+        if (variant < 0 || variant >= sph::kNumVariants)
+            return fail(SPH_ERR_INVALID, "sph_variant_shape: no variant " + std::to_string(variant));
+        if (threads) *threads = sph::kNeighbourVariants[variant].threads;
+        if (unroll) *unroll = sph::kNeighbourVariants[variant].unroll;
+        if (pairs) *pairs = sph::kNeighbourVariants[variant].pairs;
+        if (stall) *stall = sph::kNeighbourVariants[variant].stall;
+        if (queue) *queue = sph::kNeighbourVariants[variant].queue;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 // Allocates the row masks the first time the mode needs them.
@@ -1431,211 +1556,255 @@ sph_status ensure_masks(sph_ctx *c)
 
 sph_status sph_set_variant(sph_ctx *c, int variant, int mode)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (variant < 0 || variant >= sph::kNumVariants)
-        return fail(SPH_ERR_INVALID, "sph_set_variant: no variant " + std::to_string(variant));
-    if (mode < SPH_NEIGHBOURS_WALK || mode > SPH_NEIGHBOURS_MASKS)
-        return fail(SPH_ERR_INVALID, "sph_set_variant: no neighbour mode " + std::to_string(mode));
-    if (mode == SPH_NEIGHBOURS_LISTS && c->kmax == 0)
-        return fail(SPH_ERR_STATE, "sph_set_variant: the context was created without neighbour lists");
-    SPH_TRY(between_steps(c, "sph_set_variant"));
-    if (mode == SPH_NEIGHBOURS_MASKS) SPH_TRY(ensure_masks(c));
-    SPH_HIP(hipStreamSynchronize(c->stream));
-    c->variant = variant;
-    c->nb_mode = mode;
-    drop_graphs(c);
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (variant < 0 || variant >= sph::kNumVariants)
+            return fail(SPH_ERR_INVALID, "sph_set_variant: no variant " + std::to_string(variant));
+        if (mode < SPH_NEIGHBOURS_WALK || mode > SPH_NEIGHBOURS_MASKS)
+            return fail(SPH_ERR_INVALID, "sph_set_variant: no neighbour mode " + std::to_string(mode));
+        if (mode == SPH_NEIGHBOURS_LISTS && c->kmax == 0)
+            return fail(SPH_ERR_STATE, "sph_set_variant: the context was created without neighbour lists");
+        SPH_TRY(between_steps(c, "sph_set_variant"));
+        if (mode == SPH_NEIGHBOURS_MASKS) SPH_TRY(ensure_masks(c));
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        c->variant = variant;
+        c->nb_mode = mode;
+        drop_graphs(c);
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_get_variant(const sph_ctx *c, int *variant, int *mode)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (variant) *variant = c->variant;
-    if (mode) *mode = c->nb_mode;
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (variant) *variant = c->variant;
+        if (mode) *mode = c->nb_mode;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_autotune(sph_ctx *c, uint32_t steps, float *ms_out)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (steps == 0) return fail(SPH_ERR_INVALID, "sph_autotune: steps must be > 0");
-    SPH_TRY(between_steps(c, "sph_autotune"));
-    SPH_TRY(check_runnable(c, "sph_autotune"));
-    SPH_TRY(ensure_masks(c));
-    constexpr int kModes = 3;
-    if (ms_out)
-        for (int i = 0; i < kModes * sph::kNumVariants; ++i) ms_out[i] = -1.0f;
-    const uint32_t saved_flags = c->flags;
-    const long saved_fault = c->fault_at_step;
-    c->flags &= ~(uint32_t) SPH_FLAG_PROFILE;
-    c->fault_at_step = -1;
-    Snapshot snap{c};
-    sph_status st = snap.save();
-    int best_v = c->variant, best_m = c->nb_mode;
-    float best = -1.0f;
-    for (int mode = 0; mode < kModes && !st; ++mode) {
-        if (mode == SPH_NEIGHBOURS_LISTS && c->kmax == 0) continue;
-        for (int v = 0; v < sph::kNumVariants && !st; ++v) {
-            st = snap.restore();
-            if (st) break;
-            c->variant = v;
-            c->nb_mode = mode;
-            drop_graphs(c);
-            st = run_step(c);   // warm-up (and graph capture)
-            if (!st) st = run_step(c);
-            float ms = 0.0f;
-            if (!st && hipEventRecord(c->span[0], c->stream) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventRecord");
-            for (uint32_t s = 0; s < steps && !st; ++s) st = run_step(c);
-            if (!st && hipEventRecord(c->span[1], c->stream) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventRecord");
-            if (!st && hipEventSynchronize(c->span[1]) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventSynchronize");
-            if (!st && hipEventElapsedTime(&ms, c->span[0], c->span[1]) != hipSuccess)
-                st = fail(SPH_ERR_HIP, "hipEventElapsedTime");
-            if (st) break;
-            ms /= (float) steps;
-            if (ms_out) ms_out[kModes * v + mode] = ms;
-            if (best < 0.0f || ms < best) {
-                best = ms;
-                best_v = v;
-                best_m = mode;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (steps == 0) return fail(SPH_ERR_INVALID, "sph_autotune: steps must be > 0");
+        SPH_TRY(between_steps(c, "sph_autotune"));
+        SPH_TRY(check_runnable(c, "sph_autotune"));
+        SPH_TRY(ensure_masks(c));
+        constexpr int kModes = 3;
+        if (ms_out)
+            for (int i = 0; i < kModes * sph::kNumVariants; ++i) ms_out[i] = -1.0f;
+        const uint32_t saved_flags = c->flags;
+        const long saved_fault = c->fault_at_step;
+        c->flags &= ~(uint32_t) SPH_FLAG_PROFILE;
+        c->fault_at_step = -1;
+        Snapshot snap{c};
+        sph_status st = snap.save();
+        int best_v = c->variant, best_m = c->nb_mode;
+        float best = -1.0f;
+        for (int mode = 0; mode < kModes && !st; ++mode) {
+            if (mode == SPH_NEIGHBOURS_LISTS && c->kmax == 0) continue;
+            for (int v = 0; v < sph::kNumVariants && !st; ++v) {
+                st = snap.restore();
+                if (st) break;
+                c->variant = v;
+                c->nb_mode = mode;
+                drop_graphs(c);
+                st = run_step(c);   // warm-up (and graph capture)
+                if (!st) st = run_step(c);
+                float ms = 0.0f;
+                if (!st && hipEventRecord(c->span[0], c->stream) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventRecord");
+                for (uint32_t s = 0; s < steps && !st; ++s) st = run_step(c);
+                if (!st && hipEventRecord(c->span[1], c->stream) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventRecord");
+                if (!st && hipEventSynchronize(c->span[1]) != hipSuccess) st = fail(SPH_ERR_HIP, "hipEventSynchronize");
+                if (!st && hipEventElapsedTime(&ms, c->span[0], c->span[1]) != hipSuccess)
+                    st = fail(SPH_ERR_HIP, "hipEventElapsedTime");
+                if (st) break;
+                ms /= (float) steps;
+                if (ms_out) ms_out[kModes * v + mode] = ms;
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    best_v = v;
+                    best_m = mode;
+                }
             }
         }
+        sph_status st2 = snap.mem ? snap.restore() : SPH_OK;
+        c->flags = saved_flags;
+        c->fault_at_step = saved_fault;
+        c->variant = best_v;
+        c->nb_mode = best_m;
+        drop_graphs(c);
+        return st ? st : st2;
+    } catch (...) {
+        return sph::host_exception();
     }
-    sph_status st2 = snap.mem ? snap.restore() : SPH_OK;
-    c->flags = saved_flags;
-    c->fault_at_step = saved_fault;
-    c->variant = best_v;
-    c->nb_mode = best_m;
-    drop_graphs(c);
-    return st ? st : st2;
 }
 
 sph_status sph_scene_readback(sph_ctx *c, uint32_t scene, int with_velocity, const float **pos_xyz,
                               const float **vel_xyz, uint32_t *count)
 {
-    SPH_TRY(check_scene(c, scene, "sph_scene_readback"));
-    if (!pos_xyz || !count) return fail(SPH_ERR_INVALID, "sph_scene_readback: null output");
-    SPH_TRY(sync_counts(c));
-    const sph::Scene &s = c->scenes[scene].s;
-    const uint32_t n = s.n;
-    if (n > c->rb_cap || (with_velocity && !c->rb_vel)) {
-        SPH_HIP(hipStreamSynchronize(c->stream));
-        const size_t cap = std::max<size_t>(n, c->rb_cap);
-        if (c->rb_pos) (void) hipHostFree(c->rb_pos);
-        if (c->rb_vel) (void) hipHostFree(c->rb_vel);
-        c->rb_pos = c->rb_vel = nullptr;
-        c->rb_cap = 0;
-        SPH_HIP(hipHostMalloc((void **) &c->rb_pos, std::max<size_t>(cap, 1) * 12, hipHostMallocDefault));
-        if (with_velocity) SPH_HIP(hipHostMalloc((void **) &c->rb_vel, std::max<size_t>(cap, 1) * 12, hipHostMallocDefault));
-        c->rb_cap = cap;
+    try {
+        SPH_TRY(check_scene(c, scene, "sph_scene_readback"));
+        if (!pos_xyz || !count) return fail(SPH_ERR_INVALID, "sph_scene_readback: null output");
+        SPH_TRY(sync_counts(c));
+        const sph::Scene &s = c->scenes[scene].s;
+        const uint32_t n = s.n;
+        if (n > c->rb_cap || (with_velocity && !c->rb_vel)) {
+            SPH_HIP(hipStreamSynchronize(c->stream));
+            const size_t cap = std::max<size_t>(n, c->rb_cap);
+            if (c->rb_pos) (void) hipHostFree(c->rb_pos);
+            if (c->rb_vel) (void) hipHostFree(c->rb_vel);
+            c->rb_pos = c->rb_vel = nullptr;
+            c->rb_cap = 0;
+            SPH_HIP(hipHostMalloc((void **) &c->rb_pos, std::max<size_t>(cap, 1) * 12, hipHostMallocDefault));
+            if (with_velocity) SPH_HIP(hipHostMalloc((void **) &c->rb_vel, std::max<size_t>(cap, 1) * 12, hipHostMallocDefault));
+            c->rb_cap = cap;
+        }
+        if (!c->rb_bad) SPH_TRY(dalloc(c, &c->rb_bad, 1));
+        SPH_HIP(hipMemsetAsync(c->rb_bad, 0, 4, c->stream));
+        sph::launch_scatter_by_id(c->soa(c->cur, c->id_cur), s.poff, n, c->rb_pos, with_velocity ? c->rb_vel : nullptr,
+                                  c->rb_bad, c->stream);
+        SPH_TRY(launch_error("readback launch"));
+        uint32_t bad = 0;
+        SPH_TRY(copy_d2h(c, &bad, c->rb_bad, 4));
+        if (bad) return fail(SPH_ERR_STATE, "sph_scene_readback: particle ids are not 0..count-1");
+        *pos_xyz = c->rb_pos;
+        if (vel_xyz) *vel_xyz = with_velocity ? c->rb_vel : nullptr;
+        *count = n;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    if (!c->rb_bad) SPH_TRY(dalloc(c, &c->rb_bad, 1));
-    SPH_HIP(hipMemsetAsync(c->rb_bad, 0, 4, c->stream));
-    sph::launch_scatter_by_id(c->soa(c->cur, c->id_cur), s.poff, n, c->rb_pos, with_velocity ? c->rb_vel : nullptr,
-                              c->rb_bad, c->stream);
-    SPH_TRY(launch_error("readback launch"));
-    uint32_t bad = 0;
-    SPH_TRY(copy_d2h(c, &bad, c->rb_bad, 4));
-    if (bad) return fail(SPH_ERR_STATE, "sph_scene_readback: particle ids are not 0..count-1");
-    *pos_xyz = c->rb_pos;
-    if (vel_xyz) *vel_xyz = with_velocity ? c->rb_vel : nullptr;
-    *count = n;
-    return SPH_OK;
 }
 
 sph_status sph_set_event_stride(sph_ctx *c, uint32_t stride)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    c->event_stride = stride;
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        c->event_stride = stride;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_profile_events(sph_ctx *c, sph_profile_event *out, uint32_t cap, uint32_t *n, int reset)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (!(c->flags & SPH_FLAG_PROFILE)) return fail(SPH_ERR_STATE, "sph_profile_events: needs SPH_FLAG_PROFILE");
-    SPH_TRY(drain_profile(c));
-    if (out)
-        for (size_t i = 0; i < c->events.size() && i < cap; ++i) out[i] = c->events[i];
-    if (n) *n = (uint32_t) c->events.size();
-    if (reset) c->events.clear();
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (!(c->flags & SPH_FLAG_PROFILE)) return fail(SPH_ERR_STATE, "sph_profile_events: needs SPH_FLAG_PROFILE");
+        SPH_TRY(drain_profile(c));
+        if (out)
+            for (size_t i = 0; i < c->events.size() && i < cap; ++i) out[i] = c->events[i];
+        if (n) *n = (uint32_t) c->events.size();
+        if (reset) c->events.clear();
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_write_logs(const sph_profile_event *ev, uint32_t n, const char *stem, const char *label)
 {
-    if (!stem || !label || (n && !ev)) return fail(SPH_ERR_INVALID, "sph_write_logs: null argument");
-    const std::string base(stem);
-    FILE *sum = std::fopen((base + ".csv").c_str(), "a");
-    if (!sum) return fail(SPH_ERR_INVALID, "sph_write_logs: cannot open " + base + ".csv");
-    FILE *det = std::fopen((base + "_detail.csv").c_str(), "a");
-    if (!det) {
-        std::fclose(sum);
-        return fail(SPH_ERR_INVALID, "sph_write_logs: cannot open " + base + "_detail.csv");
-    }
-    // summary: the label, then every event's together() (src/mainwindow.cpp:329-345)
-    std::fprintf(sum, "%s;", label);
-    for (uint32_t i = 0; i < n; ++i)
-        std::fprintf(sum, "%g;", ev[i].update_grid + ev[i].update_density_pressure + ev[i].update_forces +
-                                      ev[i].update_collisions + ev[i].integrate);
-    std::fprintf(sum, "\n");
-    // detail: the label, one line per phase, two empty lines (src/mainwindow.cpp:347-362)
-    std::fprintf(det, "%s\n", label);
-    const char *names[5] = {"Grid", "Density + pressure", "Forces", "Collisions", "Integrate"};
-    for (int ph = 0; ph < 5; ++ph) {
-        std::fprintf(det, "%s;", names[ph]);
-        for (uint32_t i = 0; i < n; ++i) {
-            const double v = ph == 0 ? ev[i].update_grid : ph == 1 ? ev[i].update_density_pressure
-                           : ph == 2 ? ev[i].update_forces : ph == 3 ? ev[i].update_collisions : ev[i].integrate;
-            std::fprintf(det, "%g;", v);
+    try {
+        if (!stem || !label || (n && !ev)) return fail(SPH_ERR_INVALID, "sph_write_logs: null argument");
+        const std::string base(stem);
+        FILE *sum = std::fopen((base + ".csv").c_str(), "a");
+        if (!sum) return fail(SPH_ERR_INVALID, "sph_write_logs: cannot open " + base + ".csv");
+        FILE *det = std::fopen((base + "_detail.csv").c_str(), "a");
+        if (!det) {
+            std::fclose(sum);
+            return fail(SPH_ERR_INVALID, "sph_write_logs: cannot open " + base + "_detail.csv");
         }
-        std::fprintf(det, "\n");
+        // summary: the label, then every event's together() (src/mainwindow.cpp:329-345)
+        std::fprintf(sum, "%s;", label);
+        for (uint32_t i = 0; i < n; ++i)
+            std::fprintf(sum, "%g;", ev[i].update_grid + ev[i].update_density_pressure + ev[i].update_forces +
+                                          ev[i].update_collisions + ev[i].integrate);
+        std::fprintf(sum, "\n");
+        // detail: the label, one line per phase, two empty lines (src/mainwindow.cpp:347-362)
+        std::fprintf(det, "%s\n", label);
+        const char *names[5] = {"Grid", "Density + pressure", "Forces", "Collisions", "Integrate"};
+        for (int ph = 0; ph < 5; ++ph) {
+            std::fprintf(det, "%s;", names[ph]);
+            for (uint32_t i = 0; i < n; ++i) {
+                const double v = ph == 0 ? ev[i].update_grid : ph == 1 ? ev[i].update_density_pressure
+                               : ph == 2 ? ev[i].update_forces : ph == 3 ? ev[i].update_collisions : ev[i].integrate;
+                std::fprintf(det, "%g;", v);
+            }
+            std::fprintf(det, "\n");
+        }
+        std::fprintf(det, "\n\n");
+        const bool ok = std::fclose(sum) == 0 && std::fclose(det) == 0;
+        return ok ? SPH_OK : fail(SPH_ERR_INVALID, "sph_write_logs: write failed");
+    } catch (...) {
+        return sph::host_exception();
     }
-    std::fprintf(det, "\n\n");
-    const bool ok = std::fclose(sum) == 0 && std::fclose(det) == 0;
-    return ok ? SPH_OK : fail(SPH_ERR_INVALID, "sph_write_logs: write failed");
 }
 
 sph_status sph_export_logs(sph_ctx *c, const char *stem, const char *label)
 {
-    uint32_t n = 0;
-    SPH_TRY(sph_profile_events(c, nullptr, 0, &n, 0));
-    return sph_write_logs(c->events.data(), (uint32_t) c->events.size(), stem, label);
+    try {
+        uint32_t n = 0;
+        SPH_TRY(sph_profile_events(c, nullptr, 0, &n, 0));
+        return sph_write_logs(c->events.data(), (uint32_t) c->events.size(), stem, label);
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_device_name(char *buf, size_t len)
 {
-    if (!buf || len == 0) return fail(SPH_ERR_INVALID, "sph_device_name: empty buffer");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SPH_ERR_NO_DEVICE, "no HIP device visible");
-    int dev = 0;
-    SPH_HIP(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    SPH_HIP(hipGetDeviceProperties(&prop, dev));
-    std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
-    return SPH_OK;
+    try {
+        if (!buf || len == 0) return fail(SPH_ERR_INVALID, "sph_device_name: empty buffer");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SPH_ERR_NO_DEVICE, "no HIP device visible");
+        int dev = 0;
+        SPH_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        SPH_HIP(hipGetDeviceProperties(&prop, dev));
+        std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_sync(sph_ctx *c)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    SPH_HIP(hipStreamSynchronize(c->stream));
-    return SPH_OK;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_stage_times(sph_ctx *c, float *ms, uint32_t *launches, int reset)
 {
-    if (!check_ctx(c)) return SPH_ERR_INVALID;
-    if (!(c->flags & SPH_FLAG_PROFILE)) return fail(SPH_ERR_STATE, "sph_stage_times: needs SPH_FLAG_PROFILE");
-    SPH_TRY(drain_profile(c));
-    for (int i = 0; i < SPH_STAGE_COUNT; ++i) {
-        if (ms) ms[i] = c->stage_ms[i];
-        if (launches) launches[i] = c->stage_launches[i];
-        if (reset) {
-            c->stage_ms[i] = 0.0f;
-            c->stage_launches[i] = 0;
+    try {
+        if (!check_ctx(c)) return SPH_ERR_INVALID;
+        if (!(c->flags & SPH_FLAG_PROFILE)) return fail(SPH_ERR_STATE, "sph_stage_times: needs SPH_FLAG_PROFILE");
+        SPH_TRY(drain_profile(c));
+        for (int i = 0; i < SPH_STAGE_COUNT; ++i) {
+            if (ms) ms[i] = c->stage_ms[i];
+            if (launches) launches[i] = c->stage_launches[i];
+            if (reset) {
+                c->stage_ms[i] = 0.0f;
+                c->stage_launches[i] = 0;
+            }
         }
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    return SPH_OK;
 }
 
 const char *sph_last_error(void)

@@ -47,6 +47,7 @@ struct Job {
 
 namespace sph {
 sph_status report(sph_status code, const std::string &msg);   // sets sph_last_error() (sph_capi.hip)
+sph_status host_exception() noexcept;                          // the status of a caught C++ exception
 }
 using sph::report;
 
@@ -59,6 +60,7 @@ struct sph_server {
     std::condition_variable cv_work;   // wakes the scheduler
     std::condition_variable cv_done;   // wakes waiters
     bool stop = false;
+    std::string broken;   // why the scheduler stopped on a host error (empty: running)
     uint64_t next_id = 1;
     std::deque<Job *> queue;
     std::vector<Job *> running;
@@ -68,6 +70,7 @@ struct sph_server {
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
+    void run_loop();
 };
 
 namespace {
@@ -143,7 +146,33 @@ void retire(sph_server *s, Job *j)
 
 }  // namespace
 
+// The scheduler thread. A C++ exception in it (a host allocation failure) fails
+// every queued and running job and stops the server instead of ending the process.
 void sph_server::run()
+{
+    try {
+        run_loop();
+    } catch (...) {
+        (void) sph::host_exception();
+        std::lock_guard<std::mutex> g(mu);
+        broken = std::string("the scheduler stopped: ") + sph_last_error();
+        for (Job *j : queue) {
+            j->state = SPH_JOB_FAILED;
+            j->error = broken;
+            st.failed++;
+        }
+        for (Job *j : running) {
+            j->state = SPH_JOB_FAILED;
+            j->error = broken;
+            st.failed++;
+        }
+        queue.clear();
+        running.clear();
+        cv_done.notify_all();
+    }
+}
+
+void sph_server::run_loop()
 {
     std::unique_lock<std::mutex> lk(mu);
     const uint32_t chunk = prm.chunk_steps ? prm.chunk_steps : 8u;
@@ -212,76 +241,89 @@ extern "C" {
 
 sph_status sph_server_create(const sph_server_params *p, sph_server **out)
 {
-    if (!p || !out) return report(SPH_ERR_INVALID, "sph_server_create: null argument");
-    *out = nullptr;
-    if (p->max_jobs == 0) return report(SPH_ERR_INVALID, "sph_server_create: max_jobs must be > 0");
-    const uint64_t per = sph_slot_bytes(p->max_neighbours);
-    const uint64_t slots = std::min<uint64_t>(p->hbm_budget / per, 1ull << 31);
-    if (slots < 256) return report(SPH_ERR_INVALID, "sph_server_create: budget below 256 slots");
-    auto s = std::make_unique<sph_server>();
-    s->prm = *p;
-    sph_batch_params bp{};
-    bp.pool = (uint32_t) slots;
-    bp.max_scenes = p->max_jobs;
-    bp.flags = p->flags;
-    bp.max_neighbours = p->max_neighbours;
-    sph_status st = sph_batch_create(&bp, nullptr, 0, &s->ctx);
-    if (st) return st;
-    if (p->variant >= 0 && (st = sph_set_variant(s->ctx, p->variant, 0))) {
-        sph_destroy(s->ctx);
-        return st;
+    try {
+        if (!p || !out) return report(SPH_ERR_INVALID, "sph_server_create: null argument");
+        *out = nullptr;
+        if (p->max_jobs == 0) return report(SPH_ERR_INVALID, "sph_server_create: max_jobs must be > 0");
+        const uint64_t per = sph_slot_bytes(p->max_neighbours);
+        const uint64_t slots = std::min<uint64_t>(p->hbm_budget / per, 1ull << 31);
+        if (slots < 256) return report(SPH_ERR_INVALID, "sph_server_create: budget below 256 slots");
+        auto s = std::make_unique<sph_server>();
+        s->prm = *p;
+        sph_batch_params bp{};
+        bp.pool = (uint32_t) slots;
+        bp.max_scenes = p->max_jobs;
+        bp.flags = p->flags;
+        bp.max_neighbours = p->max_neighbours;
+        sph_status st = sph_batch_create(&bp, nullptr, 0, &s->ctx);
+        if (st) return st;
+        if (p->variant >= 0 && (st = sph_set_variant(s->ctx, p->variant, 0))) {
+            sph_destroy(s->ctx);
+            return st;
+        }
+        if (p->graphs) (void) sph_set_graphs(s->ctx, 1);
+        s->pool = (uint32_t) slots;
+        s->st.pool = s->pool;
+        s->st.free_slots = s->pool;
+        sph_server *raw = s.release();
+        raw->worker = std::thread([raw] { raw->run(); });
+        *out = raw;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    if (p->graphs) (void) sph_set_graphs(s->ctx, 1);
-    s->pool = (uint32_t) slots;
-    s->st.pool = s->pool;
-    s->st.free_slots = s->pool;
-    sph_server *raw = s.release();
-    raw->worker = std::thread([raw] { raw->run(); });
-    *out = raw;
-    return SPH_OK;
 }
 
 void sph_server_destroy(sph_server *s)
 {
-    if (!s) return;
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        s->stop = true;
+    try {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        if (s->worker.joinable()) s->worker.join();
+        sph_destroy(s->ctx);
+        delete s;
+    } catch (...) {
+        (void) sph::host_exception();
     }
-    s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
-    sph_destroy(s->ctx);
-    delete s;
 }
 
 sph_status sph_server_submit(sph_server *s, const sph_job_desc *d, uint64_t *job_id)
 {
-    if (!s || !d || !job_id) return report(SPH_ERR_INVALID, "sph_server_submit: null argument");
-    if (d->n > 0 && !d->pos3) return report(SPH_ERR_INVALID, "sph_server_submit: null positions");
-    if (d->n > d->scene.capacity) return report(SPH_ERR_INVALID, "sph_server_submit: n above the capacity");
-    if (d->steps == 0) return report(SPH_ERR_INVALID, "sph_server_submit: steps must be > 0");
-    if ((uint64_t) d->scene.capacity > s->pool)
-        return report(SPH_ERR_CAPACITY, "sph_server_submit: capacity " + std::to_string(d->scene.capacity) +
-                                            " above the pool of " + std::to_string(s->pool) + " slots");
-    auto j = std::make_unique<Job>();
-    j->desc = d->scene;
-    j->n = d->n;
-    j->steps = d->steps;
-    j->pos.assign(d->pos3, d->pos3 + 3 * (size_t) d->n);
-    if (d->vel3) j->vel.assign(d->vel3, d->vel3 + 3 * (size_t) d->n);
-    if (d->emitter) {
-        j->has_emitter = true;
-        j->em = *d->emitter;
+    try {
+        if (!s || !d || !job_id) return report(SPH_ERR_INVALID, "sph_server_submit: null argument");
+        if (d->n > 0 && !d->pos3) return report(SPH_ERR_INVALID, "sph_server_submit: null positions");
+        if (d->n > d->scene.capacity) return report(SPH_ERR_INVALID, "sph_server_submit: n above the capacity");
+        if (d->steps == 0) return report(SPH_ERR_INVALID, "sph_server_submit: steps must be > 0");
+        if ((uint64_t) d->scene.capacity > s->pool)
+            return report(SPH_ERR_CAPACITY, "sph_server_submit: capacity " + std::to_string(d->scene.capacity) +
+                                                " above the pool of " + std::to_string(s->pool) + " slots");
+        auto j = std::make_unique<Job>();
+        j->desc = d->scene;
+        j->n = d->n;
+        j->steps = d->steps;
+        j->pos.assign(d->pos3, d->pos3 + 3 * (size_t) d->n);
+        if (d->vel3) j->vel.assign(d->vel3, d->vel3 + 3 * (size_t) d->n);
+        if (d->emitter) {
+            j->has_emitter = true;
+            j->em = *d->emitter;
+        }
+        std::lock_guard<std::mutex> g(s->mu);
+        if (!s->broken.empty()) return report(SPH_ERR_STATE, "sph_server_submit: " + s->broken);
+        j->id = s->next_id++;
+        j->submit_s = s->now();
+        *job_id = j->id;
+        s->queue.push_back(j.get());
+        s->jobs[j->id] = std::move(j);
+        s->st.submitted++;
+        s->cv_work.notify_all();
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
     }
-    std::lock_guard<std::mutex> g(s->mu);
-    j->id = s->next_id++;
-    j->submit_s = s->now();
-    *job_id = j->id;
-    s->queue.push_back(j.get());
-    s->jobs[j->id] = std::move(j);
-    s->st.submitted++;
-    s->cv_work.notify_all();
-    return SPH_OK;
 }
 
 static void fill_info(const Job &j, sph_job_info *info)
@@ -295,57 +337,73 @@ static void fill_info(const Job &j, sph_job_info *info)
 
 sph_status sph_server_job(sph_server *s, uint64_t id, sph_job_info *info)
 {
-    if (!s || !info) return report(SPH_ERR_INVALID, "sph_server_job: null argument");
-    std::lock_guard<std::mutex> g(s->mu);
-    auto it = s->jobs.find(id);
-    if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_job: unknown job " + std::to_string(id));
-    fill_info(*it->second, info);
-    return SPH_OK;
+    try {
+        if (!s || !info) return report(SPH_ERR_INVALID, "sph_server_job: null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_job: unknown job " + std::to_string(id));
+        fill_info(*it->second, info);
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_server_wait(sph_server *s, uint64_t id, double timeout_s, sph_job_info *info)
 {
-    if (!s) return report(SPH_ERR_INVALID, "sph_server_wait: null server");
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(id);
-    if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_wait: unknown job " + std::to_string(id));
-    Job *j = it->second.get();
-    auto done = [&] { return j->state == SPH_JOB_DONE || j->state == SPH_JOB_FAILED; };
-    if (timeout_s < 0)
-        s->cv_done.wait(lk, done);
-    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))
-        return report(SPH_ERR_STATE, "sph_server_wait: timed out");
-    if (info) fill_info(*j, info);
-    if (j->state == SPH_JOB_FAILED) return report(SPH_ERR_FAULT, "job " + std::to_string(id) + " failed: " + j->error);
-    return SPH_OK;
+    try {
+        if (!s) return report(SPH_ERR_INVALID, "sph_server_wait: null server");
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_wait: unknown job " + std::to_string(id));
+        Job *j = it->second.get();
+        auto done = [&] { return j->state == SPH_JOB_DONE || j->state == SPH_JOB_FAILED; };
+        if (timeout_s < 0)
+            s->cv_done.wait(lk, done);
+        else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))
+            return report(SPH_ERR_STATE, "sph_server_wait: timed out");
+        if (info) fill_info(*j, info);
+        if (j->state == SPH_JOB_FAILED) return report(SPH_ERR_FAULT, "job " + std::to_string(id) + " failed: " + j->error);
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_server_result(sph_server *s, uint64_t id, float *pos3, float *vel3, uint32_t n)
 {
-    if (!s) return report(SPH_ERR_INVALID, "sph_server_result: null server");
-    std::lock_guard<std::mutex> g(s->mu);
-    auto it = s->jobs.find(id);
-    if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_result: unknown job " + std::to_string(id));
-    Job &j = *it->second;
-    if (j.state != SPH_JOB_DONE) return report(SPH_ERR_STATE, "sph_server_result: job not done");
-    if (n != j.n) return report(SPH_ERR_INVALID, "sph_server_result: n = " + std::to_string(n) + ", job holds " +
-                                                    std::to_string(j.n));
-    if (pos3) std::memcpy(pos3, j.pos.data(), j.pos.size() * sizeof(float));
-    if (vel3) std::memcpy(vel3, j.vel.data(), j.vel.size() * sizeof(float));
-    s->jobs.erase(it);
-    return SPH_OK;
+    try {
+        if (!s) return report(SPH_ERR_INVALID, "sph_server_result: null server");
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_result: unknown job " + std::to_string(id));
+        Job &j = *it->second;
+        if (j.state != SPH_JOB_DONE) return report(SPH_ERR_STATE, "sph_server_result: job not done");
+        if (n != j.n) return report(SPH_ERR_INVALID, "sph_server_result: n = " + std::to_string(n) + ", job holds " +
+                                                        std::to_string(j.n));
+        if (pos3) std::memcpy(pos3, j.pos.data(), j.pos.size() * sizeof(float));
+        if (vel3) std::memcpy(vel3, j.vel.data(), j.vel.size() * sizeof(float));
+        s->jobs.erase(it);
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 sph_status sph_server_stats_get(sph_server *s, sph_server_stats *out)
 {
-    if (!s || !out) return report(SPH_ERR_INVALID, "sph_server_stats_get: null argument");
-    std::lock_guard<std::mutex> g(s->mu);
-    *out = s->st;
-    out->queued = s->queue.size();
-    out->running = s->running.size();
-    out->elapsed_s = s->now();
-    out->pool = s->pool;
-    return SPH_OK;
+    try {
+        if (!s || !out) return report(SPH_ERR_INVALID, "sph_server_stats_get: null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        *out = s->st;
+        out->queued = s->queue.size();
+        out->running = s->running.size();
+        out->elapsed_s = s->now();
+        out->pool = s->pool;
+        return SPH_OK;
+    } catch (...) {
+        return sph::host_exception();
+    }
 }
 
 }  // extern "C"

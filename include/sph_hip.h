@@ -36,7 +36,7 @@ enum {
     SPH_ERR_INVALID = 1,    /* bad argument */
     SPH_ERR_CAPACITY = 2,   /* particle count beyond the context's capacity */
     SPH_ERR_HIP = 3,        /* a HIP runtime call failed */
-    SPH_ERR_OOM = 4,        /* device allocation failed */
+    SPH_ERR_OOM = 4,        /* device (or host) allocation failed */
     SPH_ERR_NO_DEVICE = 5,  /* no usable gfx950 device */
     SPH_ERR_STATE = 6,      /* phase called out of order / nothing uploaded */
     SPH_ERR_FAULT = 7       /* injected failure (SPH_FAULT_AT_STEP env hook) */
