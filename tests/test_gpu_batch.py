@@ -446,3 +446,42 @@ def test_profile_events_and_export(hip, tmp_path):
         assert rows[0] == "HIP Grid" and rows[1].startswith("Grid;") and rows[5].startswith("Integrate;")
         assert len(rows[2].rstrip(";").split(";")) == 4                   # label + 3 events
         sim.close()
+
+
+def test_full_rate_fountain_emitter_matches_host_append(hip, orc):
+    """The fountain workload's emitter at its full rate (1,024 particles per step
+    on a disc of radius 8h in the 1.6 m box), with graph replay: bitwise a host
+    append of the oracle's restatement, for 12 steps (12,288 particles)."""
+    f = hip.scenes.FOUNTAIN
+    em = dict(rate=f["rate"], limit=f["limit"], centre=(0.0, 0.0), radius=f["radius"], speed=f["speed"],
+              spread=f["spread"], seed=f["seed"])
+    steps = 12
+    with hip.SceneBatch([(f["box"], f["limit"])]) as b:
+        b.set_emitter(0, kind=hip._lib.EMIT_DISC, **em)
+        b.set_graphs(True)
+        b.step(steps)
+        p, v, _ = b.download(0)
+    (pr, vr, _), n = disc_host_reference(hip, orc, f["box"], em, steps)
+    assert len(p) == n == steps * f["rate"]
+    assert_bits_equal(p, pr, "fountain positions")
+    assert_bits_equal(v, vr, "fountain velocities")
+
+
+def test_full_batch_scenes_equal_alone(hip):
+    """The 512-scene batched workload (9.65 M particles), 3 steps: every scene
+    keeps its particles (ids a permutation of 0..n-1) and sampled scenes are
+    bitwise their solo runs."""
+    ws = hip.scenes.batch_workload()
+    with hip.SceneBatch([(s["box"], s["n"]) for s in ws]) as b:
+        for i, s in enumerate(ws):
+            b.upload(i, s["pos"])
+        b.step(3)
+        for i, s in enumerate(ws):
+            ids = b.debug(i, hip._lib.DEBUG_IDS)
+            assert np.array_equal(np.sort(ids), np.arange(s["n"], dtype=np.uint32)), "scene %d ids" % i
+        for i in (0, 1, 100, 257, 511):
+            p, v, _ = b.download(i)
+            pa, va, _ = run_alone(hip, ws[i], 3)
+            assert_bits_equal(p, pa, "scene %d positions" % i)
+            assert_bits_equal(v, va, "scene %d velocities" % i)
+
