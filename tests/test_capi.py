@@ -63,6 +63,23 @@ def test_argument_validation_before_device(hip_host):
     lib.sph_destroy(None)
 
 
+def test_variant_table_matches_header_and_binding(hip_host):
+    """SPH_NUM_VARIANTS in the header, the binding's NUM_VARIANTS and the library's
+    table agree: every index below it has a shape, the next one is rejected, and
+    the default is a word-queue shape."""
+    lib, _lib = hip_host
+    n = int(re.search(r"#define SPH_NUM_VARIANTS (\d+)", open(HEADER).read()).group(1))
+    assert _lib.NUM_VARIANTS == n
+    vals = [C.c_int() for _ in range(5)]
+    for v in range(n):
+        assert lib.sph_variant_shape(v, *[C.byref(x) for x in vals]) == _lib.SPH_OK, v
+        threads, unroll, pairs, stall, queue = (x.value for x in vals)
+        assert threads in (64, 128, 256) and unroll in (2, 4, 8) and pairs >= 16 and queue in (0, 1)
+    assert lib.sph_variant_shape(n, *[C.byref(x) for x in vals]) == _lib.SPH_ERR_INVALID
+    assert lib.sph_variant_shape(_lib.DEFAULT_VARIANT, *[C.byref(x) for x in vals]) == _lib.SPH_OK
+    assert vals[4].value == 1
+
+
 def test_default_params_are_the_reference_constants(hip_host):
     lib, _lib = hip_host
     p = _lib.sph_params()
