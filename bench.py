@@ -13,7 +13,7 @@ The same line carries the batched and served forms of the step:
   * "batched": 512 independent scenes (2,048-65,536 particles each, log-uniform,
     seed 3) advanced together for 500 steps (autotuned kernel shape, HIP graph
     replay), against the same scenes stepped by plain per-scene launches (one
-    context and stream per scene, a bounded sample of steps);
+    context and stream per scene); both time steps 4-503 from the same start;
   * "fountain": a disc emitter adding 1,024 particles per step in the 1.6 m box
     until 524,288, 2,000 steps with the on-device emitter, and its growth phase
     against host-side generation plus a tail upload before every step;
@@ -108,8 +108,11 @@ def load_traffic(workload):
         return None
 
 
-def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
-    """The batched workload: one context for all scenes vs one context per scene."""
+def bench_batched(sphhip, steps, per_scene_steps=None, nscenes=512, seed=3):
+    """The batched workload: one context for all scenes vs one context per scene.
+    Both time steps 4 onwards of the same scenes (the autotuner restores the
+    state it stepped); per_scene_steps None times the per-scene run over the
+    same window as the batched one."""
     from sphhip import _lib
     lib = _lib.load()
     w = sphhip.scenes.batch_workload(nscenes=nscenes, seed=seed)
@@ -132,9 +135,12 @@ def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
                        "boxes at 1/4 fill with per-axis aspect U[0.5, 2]" % (nscenes, seed),
            "scenes": nscenes, "particles": total, "steps": steps, "value": total * steps / wall,
            "unit": "particle-steps/s", "ms_per_step": wall * 1e3 / steps, "graphs": True,
+           "timed_steps": "4-%d" % (steps + 3),
            "variant": {"index": variant, "threads_unroll_pairs_stall_queue": sphhip.variant_shape(variant),
                        "neighbours": ("walk", "lists", "masks")[mode]},
            "autotune_ms_per_step": {"%d%s" % (v, "wlm"[m]): round(ms, 4) for (v, m), ms in tune.items()}}
+    if per_scene_steps is None:
+        per_scene_steps = steps
     if per_scene_steps:
         sims = []
         for s in w:
@@ -142,7 +148,7 @@ def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
             sim.upload(s["pos"])
             sims.append(sim)
         for sim in sims:
-            lib.sph_step(sim._ctx, 2, None)
+            _lib.check(lib.sph_step(sim._ctx, 4, None))
         for sim in sims:
             sim.sync()
         t0 = time.perf_counter()
@@ -157,7 +163,7 @@ def bench_batched(sphhip, steps, per_scene_steps, nscenes=512, seed=3):
         ps = total * per_scene_steps / pw
         out["per_scene"] = {"value": ps, "ms_per_step": pw * 1e3 / per_scene_steps, "steps": per_scene_steps,
                             "how": "one context and stream per scene, sph_step(ctx, 1) per scene per step from one "
-                                   "host thread, steps 2-%d" % (per_scene_steps + 1)}
+                                   "host thread, steps 4-%d" % (per_scene_steps + 3)}
         out["speedup_vs_per_scene"] = out["value"] / ps
     return out
 
@@ -259,7 +265,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=500, help="0 skips the batched measurement")
-    ap.add_argument("--per-scene-steps", type=int, default=60)
+    ap.add_argument("--per-scene-steps", type=int, default=None,
+                    help="steps of the per-scene comparison (default: --batched-steps; 0 skips it)")
     ap.add_argument("--fountain-steps", type=int, default=2000, help="0 skips the fountain measurement")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="0 skips the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
