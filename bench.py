@@ -200,6 +200,29 @@ def bench_fountain(sphhip, steps, host_steps):
            "unit": "particle-steps/s", "ms_per_step": wall * 1e3 / steps, "graphs": True,
            "growth": {"steps": host_steps, "device_emitter_ms_per_step": grow * 1e3 / host_steps}}
 
+    # Per-stage device times from HIP events on the library's stream, in a
+    # separate profiled context (eager launches), over the steps after the pool
+    # is full, where the particle count is constant.
+    full = -(-limit // rate)
+    if steps > full:
+        pb = sphhip.SceneBatch([(box, limit)], flags=_lib.FLAG_PROFILE)
+        pb.set_emitter(0, kind=_lib.EMIT_DISC, rate=rate, limit=limit, centre=(0.0, 0.0), radius=f["radius"],
+                       speed=f["speed"], spread=f["spread"], seed=f["seed"])
+        pb.step(full)
+        pb.sync()
+        pb.stage_times(reset=True)
+        pb.step(steps - full)
+        pb.sync()
+        st = pb.stage_times()
+        pb.close()
+        stages = {k: ms / cnt for k, (ms, cnt) in st.items() if cnt}
+        dominant = max(("density", "forces"), key=lambda k: stages.get(k, 0.0))
+        out["stage_ms"] = stages
+        out["stage_how"] = ("HIP events per stage in a second, profiled context with eager launches, steps %d-%d "
+                            "(%d particles)" % (full, steps - 1, limit))
+        out["dominant"] = {"kernel": dominant, "ms": stages[dominant],
+                           "alg_GBs": ALG_BYTES[dominant] * limit / (stages[dominant] * 1e-3) / 1e9}
+
     # Host-side injection: the same number of particles per step drawn on the
     # host (uniform on the disc), appended before each step.
     rng = np.random.default_rng(f["seed"])

@@ -176,6 +176,14 @@ class SceneBatch:
     def set_graphs(self, enable):
         _lib.check(self._lib.sph_set_graphs(self._ctx, int(bool(enable))))
 
+    def stage_times(self, reset=False):
+        """{stage: (device ms summed, launches)} since the last reset, for all
+        scenes together (a context made with FLAG_PROFILE)."""
+        ms = np.zeros(8, np.float32)
+        cnt = np.zeros(8, np.uint32)
+        _lib.check(self._lib.sph_stage_times(self._ctx, _lib.ptr(ms), _lib.ptr(cnt), int(reset)))
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(_lib.STAGES)}
+
     def set_variant(self, variant, mode=_lib.NEIGHBOURS_WALK):
         """Kernel shape `variant` and neighbour mode (NEIGHBOURS_WALK, _LISTS, _MASKS)."""
         _lib.check(self._lib.sph_set_variant(self._ctx, int(variant), int(mode)))

@@ -448,6 +448,29 @@ def test_profile_events_and_export(hip, tmp_path):
         sim.close()
 
 
+def test_batch_stage_times(hip):
+    """Per-stage HIP-event times of a batch context: one launch of each step
+    stage per step for all scenes together, cleared by a reset; a context
+    without FLAG_PROFILE refuses."""
+    lib = hip._lib
+    scenes = [hip.scenes.lattice(n, n, n, 0.4, seed=s) for n, s in ((8, 1), (11, 2))]
+    with hip.SceneBatch([(0.4, len(p)) for p in scenes], flags=lib.FLAG_PROFILE) as b:
+        for i, p in enumerate(scenes):
+            b.upload(i, p)
+        b.step(3)
+        b.stage_times(reset=True)
+        b.step(5)
+        st = b.stage_times()
+        for name in ("sort", "cells", "reorder", "density", "forces"):
+            ms, cnt = st[name]
+            assert cnt == 5 and ms > 0, (name, st[name])
+        assert b.stage_times(reset=True) == st
+        assert all(v == (0.0, 0) for v in b.stage_times().values())
+    with hip.SceneBatch([(0.4, len(scenes[0]))]) as b:
+        with pytest.raises(lib.SphError):
+            b.stage_times()
+
+
 def test_full_rate_fountain_emitter_matches_host_append(hip, orc):
     """The fountain workload's emitter at its full rate (1,024 particles per step
     on a disc of radius 8h in the 1.6 m box), with graph replay: bitwise a host
