@@ -40,6 +40,7 @@ SIGNATURES = {
     "sph_set_gravity": (C.c_int, [_vp, _f, _f, _f]),
     "sph_phase": (C.c_int, [_vp, C.c_int, _vp]),
     "sph_step": (C.c_int, [_vp, _u32, _vp]),
+    "sph_step_noalloc": (C.c_int, [_vp, _vp, _u32, _u32, _vp]),
     "sph_download": (C.c_int, [_vp, _vp, _vp, _vp, _u32, C.c_int]),
     "sph_debug": (C.c_int, [_vp, C.c_int, _vp, C.c_size_t]),
     "sph_count": (C.c_int, [_vp, C.POINTER(_u32)]),
@@ -88,6 +89,14 @@ SIGNATURES = {
 class sph_params(C.Structure):
     _fields_ = [("box", C.c_float * 3), ("gravity", C.c_float * 3), ("dt", C.c_float),
                 ("capacity", C.c_uint32), ("flags", C.c_uint32), ("max_neighbours", C.c_uint32)]
+
+
+class sph_device_state(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("px", "py", "pz", "vx", "vy", "vz", "id")]
+
+
+NOALLOC_MAX_PARTICLES = 65536
+NOALLOC_MAX_CELLS = 262144
 
 
 class sph_scene_desc(C.Structure):

@@ -134,6 +134,31 @@ sph_status sph_phase(sph_ctx *ctx, int phase, float *ms_out);
  * If ms_out is not NULL the call waits and stores the device time of all nsteps in ms. */
 sph_status sph_step(sph_ctx *ctx, uint32_t nsteps, float *ms_out);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the steps of sph_step on particle arrays the caller owns in device memory, calling no
+ * hipMalloc/hipFree or other allocating routine. Every other buffer (the second particle
+ * buffer, sort keys and permutation, radix histograms, the cell table, density records)
+ * lives in one static device array of SPH_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends. `state` holds n particles in array order (structure of arrays, device
+ * pointers); params gives box, gravity and dt (capacity, flags and max_neighbours are
+ * not used). The call enqueues nsteps steps on `stream` (a hipStream_t; NULL = the null
+ * stream) and returns; once they have run, the arrays hold the state sph_step leaves
+ * after sph_upload of the same arrays, bit for bit, in its new array order (the ids
+ * permuted with the particles). n above SPH_NOALLOC_MAX_PARTICLES or a grid of more than
+ * SPH_NOALLOC_MAX_CELLS cells is SPH_ERR_CAPACITY. All calls on one GPU share its array:
+ * calls that overlap on one GPU (other threads or streams) race and must not be made. */
+#define SPH_NOALLOC_MAX_PARTICLES 65536u
+#define SPH_NOALLOC_MAX_CELLS 262144u
+#define SPH_NOALLOC_SCRATCH_BYTES 7357696u
+typedef struct {
+    float *px, *py, *pz;
+    float *vx, *vy, *vz;
+    uint32_t *id;
+} sph_device_state;
+sph_status sph_step_noalloc(const sph_params *params, const sph_device_state *state, uint32_t n, uint32_t nsteps,
+                            void *stream);
+
 /* Replaces: the readback in CGPUBaseParticleSimulator::integrate
  * (src/CGPUBaseParticleSimulator.cpp:83-91). Copies n (= count) particles; by_id != 0
  * writes particle id k to row k, else rows follow the array order. Any pointer may be NULL. */

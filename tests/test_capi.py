@@ -63,6 +63,29 @@ def test_argument_validation_before_device(hip_host):
     lib.sph_destroy(None)
 
 
+def test_noalloc_limits_checked_before_device(hip_host):
+    """sph_step_noalloc rejects null arrays, sizes above its stated maximum and
+    grids above its cell maximum before any device call; the header, the binding
+    and the library agree on the limits."""
+    lib, _lib = hip_host
+    text = open(HEADER).read()
+    assert int(re.search(r"#define SPH_NOALLOC_MAX_PARTICLES (\d+)u", text).group(1)) == _lib.NOALLOC_MAX_PARTICLES
+    assert int(re.search(r"#define SPH_NOALLOC_MAX_CELLS (\d+)u", text).group(1)) == _lib.NOALLOC_MAX_CELLS
+    p = _lib.sph_params()
+    lib.sph_default_params(C.byref(p), 0.4, 8)
+    st = _lib.sph_device_state()
+    assert lib.sph_step_noalloc(C.byref(p), C.byref(st), 8, 1, None) == _lib.SPH_ERR_INVALID
+    for name, _ in _lib.sph_device_state._fields_:   # never dereferenced on these paths
+        setattr(st, name, 0x1000)
+    assert lib.sph_step_noalloc(None, C.byref(st), 8, 1, None) == _lib.SPH_ERR_INVALID
+    assert lib.sph_step_noalloc(C.byref(p), C.byref(st), _lib.NOALLOC_MAX_PARTICLES + 1, 1, None) == \
+        _lib.SPH_ERR_CAPACITY
+    assert b"65536" in lib.sph_last_error()
+    lib.sph_default_params(C.byref(p), 3.0, 8)   # 66^3 = 287,496 cells
+    assert lib.sph_step_noalloc(C.byref(p), C.byref(st), 8, 1, None) == _lib.SPH_ERR_CAPACITY
+    assert b"262144" in lib.sph_last_error()
+
+
 def test_variant_table_matches_header_and_binding(hip_host):
     """SPH_NUM_VARIANTS in the header, the binding's NUM_VARIANTS and the library's
     table agree: every index below it has a shape, the next one is rejected, and
