@@ -17,6 +17,7 @@
 
 #include "sph_kernels.h"
 #include "sph_math.h"
+#include "sph_pow_tables.h"   // generated at build time (gen_pow_tables.hip)
 
 #include <climits>
 
@@ -726,6 +727,28 @@ struct MaskRecorder {
     }
 };
 
+// One term of updateDensityPressure (src/CCPUParticleSimulator.cpp:126):
+// density += Wpoly6(r^2) in double, rounded to the float density, with
+// Wpoly6 = poly6 * pow(h^2 - r^2, 3) as the C library's pow gives it
+// (src/CCPUParticleSimulator.cpp:14): the correctly rounded cube, and the pow
+// table only where the float result can depend on it (near_float_boundary).
+__device__ __forceinline__ double density_sum_lib(float density, float r2, const Consts &k)
+{
+    const double t = k.hsq - (double) r2;
+    double c = cube_cr(t);
+    const int d = pow_exception(g_pow3_exc, g_pow3_exc_count, __float_as_uint(r2));
+    if (d) c = next_double(c, d);
+    return (double) density + k.poly6 * c;
+}
+
+__device__ __forceinline__ float density_add(float density, float r2, const Consts &k)
+{
+    const double wpoly = k.poly6 * cube_cr(k.hsq - (double) r2);
+    double sum = (double) density + wpoly;
+    if (__builtin_expect(near_float_boundary(sum), 0)) sum = density_sum_lib(density, r2, k);
+    return (float) sum;
+}
+
 // updateDensityPressure, src/CCPUParticleSimulator.cpp:93-141. Also stores
 // p/rho^2 (double, as src/CCPUParticleSimulator.cpp:181 forms it), 1/rho
 // (double) and, for the force pass, with kLists the particle's accepted
@@ -772,10 +795,7 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
         return ok;
     };
     auto flush = [&]() {
-        for (int q = 0; q < cnt; ++q) {
-            const double wpoly = k.poly6 * cube_cr(k.hsq - (double) my[q * T]);
-            density = (float) ((double) density + wpoly);
-        }
+        for (int q = 0; q < cnt; ++q) density = density_add(density, my[q * T], k);
     };
     if constexpr (kMasks)
         walk_neighbours<U, false, kCap>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush, rec_masks);
@@ -846,6 +866,24 @@ struct PairTerms {
     float px, py, pz, vx, vy, vz;
 };
 
+// (float)(spiky * pow(h - r, 2)) with the C library's pow
+// (src/CCPUParticleSimulator.cpp:21): the correctly rounded square, and the pow
+// table only where the float can depend on it.
+__device__ __forceinline__ double spiky_lib(double hr, float r2, const Consts &k)
+{
+    double sq = hr * hr;
+    const int d = pow_exception(g_pow2_exc, g_pow2_exc_count, __float_as_uint(r2));
+    if (d) sq = next_double(sq, d);
+    return k.spiky * sq;
+}
+
+__device__ __forceinline__ float spiky_scale(double hr, float r2, const Consts &k)
+{
+    double sg = k.spiky * (hr * hr);
+    if (__builtin_expect(near_float_boundary(sg), 0)) sg = spiky_lib(hr, r2, k);
+    return (float) sg;
+}
+
 __device__ __forceinline__ PairTerms pair_terms(const BufView &bv, const Consts &k, const PairData &n, float xi,
                                                 float yi, float zi, float vxi, float vyi, float vzi, double ai)
 {
@@ -853,7 +891,7 @@ __device__ __forceinline__ PairTerms pair_terms(const BufView &bv, const Consts 
     const float r2 = dx * dx + dyv * dyv + dzv * dzv;
     const double r = sqrt_of_float((double) r2);
     const double hr = k.h - r;
-    const float sg = (float) (k.spiky * (hr * hr));
+    const float sg = spiky_scale(hr, r2, k);
     float gx, gy, gz;
     div3_small_divisor(dx * sg, dyv * sg, dzv * sg, (float) r, gx, gy, gz);
     const float pf = (float) (ai + n.prr);

@@ -9,8 +9,10 @@
 namespace sph {
 
 // t^3 correctly rounded (error-free products via FMA), the value libm's
-// pow(t, 3.0) returns (src/CCPUParticleSimulator.cpp:14).
-__device__ __forceinline__ double cube_cr(double t)
+// pow(t, 3.0) returns (src/CCPUParticleSimulator.cpp:14) for every t = h^2 - r^2
+// of an accepted pair (checked on the host against glibc pow for every float r^2
+// in [0, h^2], and the device results against the host's). Host and device.
+__host__ __device__ __forceinline__ double cube_cr(double t)
 {
     const double p = t * t;
     const double e = __builtin_fma(t, t, -p);   // t*t = p + e exactly
@@ -18,6 +20,51 @@ __device__ __forceinline__ double cube_cr(double t)
     const double f = __builtin_fma(p, t, -q);   // p*t = q + f exactly
     const double c = __builtin_fma(e, t, f);    // f + e*t
     return q + c;
+}
+
+// The C library's pow is not correctly rounded (an error bound of about 0.52
+// ulp): where the exact power lies within a few thousandths of an ulp of a
+// rounding midpoint it may return the other neighbour. The inputs where it does
+// are enumerated at build time over the whole domain of accepted pairs (every
+// float r^2 in [0, fl(h^2)], gen_pow_tables.hip) into sorted tables of the float
+// bit pattern of r^2, bit 31 set when pow returns the next double up (else the
+// next double down). pow_exception looks one input up: +1 / -1 / 0 (not listed).
+__device__ __forceinline__ int pow_exception(const uint32_t *tab, uint32_t n, uint32_t r2_bits)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((tab[mid] & 0x7FFFFFFFu) < r2_bits)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo < n && (tab[lo] & 0x7FFFFFFFu) == r2_bits) return (tab[lo] >> 31) ? 1 : -1;
+    return 0;
+}
+
+// The double next to a positive finite x, above (dir = +1) or below (dir = -1).
+__device__ __forceinline__ double next_double(double x, int dir)
+{
+    return __longlong_as_double(__double_as_longlong(x) + (long long) dir);
+}
+
+// Whether rounding the double x to float can change when x moves by up to 6 of
+// its ulps: only if x lies within 6 ulps of a float rounding boundary, the
+// midpoint of two adjacent floats, whose 29 low significand bits are 1 followed
+// by 28 zeros in x's binade (a boundary in the neighbouring binade is at least
+// 2^-25 x away). The window here is 16 ulps either side.
+//
+// The kernels use it to consult the pow tables only when the bits they keep can
+// depend on them. A power p replaced by its neighbour p' moves a product c*p by
+// at most 2 ulps before rounding (the relative spacing of doubles is within a
+// factor 2), so fl(c*p) by at most 4; a sum fl(D + fl(c*p)) with D >= 0 then
+// moves by at most 5.5 of its own ulps. Outside the window both choices give
+// the same float.
+__device__ __forceinline__ bool near_float_boundary(double x)
+{
+    const uint32_t lo = (uint32_t) __double_as_longlong(x);
+    return ((lo & 0x1FFFFFFFu) - (0x10000000u - 16u)) < 32u;
 }
 
 // sqrt(x), correctly rounded, for x the double of a finite non-negative float

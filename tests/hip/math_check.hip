@@ -5,11 +5,17 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
 #include "../../gmu-water-simulation_amd/csrc/sph_math.h"
+#include "sph_pow_tables.h"   // the library's generated tables (gmu-water-simulation_amd/build)
 
 namespace {
 
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -98,6 +104,54 @@ __global__ __launch_bounds__(256) void k_div(uint64_t seed, uint64_t n, unsigned
     report(local, fl, bad, first);
 }
 
+// Results for the float r^2 of bit pattern b are hashed with their input index,
+// so that a sum over all inputs is an order-free fingerprint of every result bit.
+__host__ __device__ __forceinline__ uint64_t fingerprint(double v, uint32_t b)
+{
+    uint64_t u;
+    __builtin_memcpy(&u, &v, 8);
+    return mix64(u ^ ((uint64_t) b * 0x9E3779B97F4A7C15ull));
+}
+
+// Fingerprint sums over float bit patterns lo .. lo+count-1 of the kernels' two
+// library powers, computed on the device as the kernels compute them (the
+// correctly rounded power, corrected by the generated tables):
+// pow(h^2 - r^2, 3) (Wpoly6, src/CCPUParticleSimulator.cpp:14) and pow(h - r, 2)
+// with the kernels' sqrt (WspikyGradient, src/CCPUParticleSimulator.cpp:20-21).
+__global__ __launch_bounds__(256) void k_pow_fingerprint(uint32_t lo, uint32_t count, double hsq, double h,
+                                                         unsigned long long *sums)
+{
+    unsigned long long a = 0, b = 0;
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < count; t += gridDim.x * 256u) {
+        const uint32_t bits = lo + t;
+        const double x = (double) __uint_as_float(bits);
+        double c = sph::cube_cr(hsq - x);
+        const int d3 = sph::pow_exception(sph::g_pow3_exc, sph::g_pow3_exc_count, bits);
+        if (d3) c = sph::next_double(c, d3);
+        a += fingerprint(c, bits);
+        const double hr = h - sph::sqrt_of_float(x);
+        double q = hr * hr;
+        const int d2 = sph::pow_exception(sph::g_pow2_exc, sph::g_pow2_exc_count, bits);
+        if (d2) q = sph::next_double(q, d2);
+        b += fingerprint(q, bits);
+    }
+    __shared__ unsigned long long sa[256], sb[256];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(sums, sa[0]);
+        atomicAdd(sums + 1, sb[0]);
+    }
+}
+
 int run(void (*launch)(unsigned long long *, unsigned long long *), unsigned long long *bad_out,
         unsigned long long *first_out)
 {
@@ -119,6 +173,19 @@ int run(void (*launch)(unsigned long long *, unsigned long long *), unsigned lon
 
 uint32_t g_lo, g_count;
 uint64_t g_seed, g_n;
+
+// glibc's pow, called through a pointer so that the compiler cannot replace pow(x, 2.0) by x * x.
+double (*volatile g_pow)(double, double) = std::pow;
+
+// The double next to a positive finite x (host side of sph::next_double).
+double step_double(double x, int dir)
+{
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    u += (uint64_t) (int64_t) dir;
+    std::memcpy(&x, &u, 8);
+    return x;
+}
 
 }  // namespace
 
@@ -143,6 +210,90 @@ int mc_div(uint64_t seed, uint64_t n, unsigned long long *bad, unsigned long lon
     return run([](unsigned long long *b, unsigned long long *f) {
         hipLaunchKernelGGL(k_div, dim3(8192), dim3(256), 0, 0, g_seed, g_n, b, f);
     }, bad, first);
+}
+
+// Host check against the C library's pow for the float r^2 of bit patterns
+// lo..hi (inclusive), on `threads` host threads, of the kernels' powers corrected
+// by the tables tab3 / tab2 (sorted entries: r^2 bits | 2^31 when pow is the next
+// double up): out[0] / out[1] = inputs where the corrected cube_cr(h^2 - r^2) /
+// (h - sqrt(r^2))^2 differs from pow(., 3.0) / pow(., 2.0), out[2] / out[3] = the
+// first of them (or ~0), out[4] / out[5] = fingerprint sums of the pow results
+// themselves, out[6] / out[7] = inputs where the uncorrected power differs.
+int mc_pow_host(uint32_t lo, uint32_t hi, double hsq, double h, int threads, const uint32_t *tab3, uint32_t n3,
+                const uint32_t *tab2, uint32_t n2, unsigned long long *out)
+{
+    if (hi < lo || threads < 1 || threads > 256) return 2;
+    const uint64_t count = (uint64_t) hi - lo + 1;
+    std::vector<unsigned long long> part((size_t) threads * 8, 0);
+    std::vector<std::thread> pool;
+    // first table entry whose r^2 bits are >= b
+    auto lower = [](const uint32_t *tab, uint32_t n, uint32_t b) {
+        uint32_t l = 0, r = n;
+        while (l < r) {
+            const uint32_t m = (l + r) / 2;
+            if ((tab[m] & 0x7FFFFFFFu) < b) l = m + 1; else r = m;
+        }
+        return l;
+    };
+    for (int w = 0; w < threads; ++w) {
+        pool.emplace_back([&, w]() {
+            unsigned long long *o = &part[(size_t) w * 8];
+            o[2] = o[3] = ~0ull;
+            const uint64_t a0 = count * (uint64_t) w / (uint64_t) threads, a1 = count * (uint64_t) (w + 1) / (uint64_t) threads;
+            uint32_t k3 = lower(tab3, n3, lo + (uint32_t) a0), k2 = lower(tab2, n2, lo + (uint32_t) a0);
+            for (uint64_t q = a0; q < a1; ++q) {
+                const uint32_t bits = lo + (uint32_t) q;
+                float r2f;
+                std::memcpy(&r2f, &bits, 4);
+                const double x = (double) r2f, t = hsq - x;
+                double c = sph::cube_cr(t);
+                const double cp = g_pow(t, 3.0);
+                const double hr = h - std::sqrt(x);
+                double s = hr * hr;
+                const double sp = g_pow(hr, 2.0);
+                o[6] += std::memcmp(&c, &cp, 8) != 0;
+                o[7] += std::memcmp(&s, &sp, 8) != 0;
+                if (k3 < n3 && (tab3[k3] & 0x7FFFFFFFu) == bits) c = step_double(c, (tab3[k3++] >> 31) ? 1 : -1);
+                if (k2 < n2 && (tab2[k2] & 0x7FFFFFFFu) == bits) s = step_double(s, (tab2[k2++] >> 31) ? 1 : -1);
+                if (std::memcmp(&c, &cp, 8) != 0) {
+                    ++o[0];
+                    if (o[2] == ~0ull) o[2] = bits;
+                }
+                if (std::memcmp(&s, &sp, 8) != 0) {
+                    ++o[1];
+                    if (o[3] == ~0ull) o[3] = bits;
+                }
+                o[4] += fingerprint(cp, bits);
+                o[5] += fingerprint(sp, bits);
+            }
+        });
+    }
+    for (auto &t : pool) t.join();
+    for (int q = 0; q < 8; ++q) out[q] = (q == 2 || q == 3) ? ~0ull : 0ull;
+    for (int w = 0; w < threads; ++w) {
+        const unsigned long long *o = &part[(size_t) w * 8];
+        for (int q : {0, 1, 4, 5, 6, 7}) out[q] += o[q];
+        out[2] = o[2] < out[2] ? o[2] : out[2];
+        out[3] = o[3] < out[3] ? o[3] : out[3];
+    }
+    return 0;
+}
+
+// Device fingerprint sums (out[0], out[1]) of the same results over lo..hi.
+int mc_pow_device(uint32_t lo, uint32_t hi, double hsq, double h, unsigned long long *out)
+{
+    if (hi < lo) return 2;
+    unsigned long long *d = nullptr;
+    if (hipMalloc(&d, 2 * sizeof(unsigned long long)) != hipSuccess) return 1;
+    const unsigned long long init[2] = {0ull, 0ull};
+    int rc = hipMemcpy(d, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess;
+    if (!rc) {
+        hipLaunchKernelGGL(k_pow_fingerprint, dim3(8192), dim3(256), 0, 0, lo, hi - lo + 1, hsq, h, d);
+        rc = hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess;
+    }
+    if (!rc) rc = hipMemcpy(out, d, sizeof(init), hipMemcpyDeviceToHost) != hipSuccess;
+    (void) hipFree(d);
+    return rc;
 }
 
 }  // extern "C"

@@ -4,6 +4,11 @@
   * sqrt_of_float, the force pass's double sqrt of r^2 (src/CCPUParticleSimulator.cpp:20),
     equals the compiler's correctly rounded sqrt for EVERY float r^2 in [0, h^2]
     (all 990,437,133 bit patterns): the whole input domain of an accepted pair;
+  * the kernels' library powers (the correctly rounded pow(h^2 - r^2, 3) and
+    pow(h - r, 2), corrected by the generated tables where glibc's pow differs,
+    src/CCPUParticleSimulator.cpp:14, 21) equal this host's glibc pow for EVERY
+    float r^2 in [0, h^2]: fingerprints of all device results against
+    fingerprints of all host pow results;
   * the double-reciprocal float quotients (pressure gradient / r and viscosity
     term / rho_j, src/CCPUParticleSimulator.cpp:21, 28, 181-182) equal the IEEE
     float division on 2^28 random cases per run, with numerators down to 2^-149 so
@@ -29,6 +34,7 @@ def mc():
     lib = C.CDLL(LIB)
     lib.mc_sqrt.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
     lib.mc_div.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
+    lib.mc_pow_device.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.POINTER(C.c_ulonglong)]
     return lib
 
 
@@ -44,3 +50,17 @@ def test_quotients_match_ieee_division(mc, seed):
     bad, first = C.c_ulonglong(0), C.c_ulonglong(0)
     assert mc.mc_div(seed, 1 << 28, C.byref(bad), C.byref(first)) == 0
     assert bad.value == 0, "%d mismatches, first at case %d" % (bad.value, first.value)
+
+
+def test_library_powers_equal_host_pow_exhaustive(mc):
+    """Device: cube_cr / hr*hr corrected by the library's tables, as the kernels
+    evaluate them; host: glibc pow itself, on this machine."""
+    import test_pow_tables as tp
+    tab3, tab2, _ = tp.load_tables()
+    host = tp.host_check(tab3, tab2, threads=16)
+    assert host[0] == 0 and host[1] == 0, host
+    dev = (C.c_ulonglong * 2)()
+    hsq = float(np.uint32(HSQ_BITS).view(np.float32))
+    assert mc.mc_pow_device(0, HSQ_BITS, hsq, float(np.float32(0.0457)), dev) == 0
+    assert dev[0] == host[4], "device pow(h^2 - r^2, 3) differs from the host's pow"
+    assert dev[1] == host[5], "device pow(h - r, 2) differs from the host's pow"

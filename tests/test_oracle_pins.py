@@ -270,6 +270,74 @@ def test_pair_force_closed_form(orc, k):
     assert rel(st["acc"][1, 0], -float(Decimal(pr["ax_sph"]))) < 1e-5
 
 
+def _brute_accel(pos, vel, rho, prs, g, form="reference", rows=200):
+    """All-pairs float64 restatement of updateForces without walls
+    (src/CCPUParticleSimulator.cpp:155-193), from given densities and pressures:
+    a_i = (-m rho_i sum_j (p_i/rho_i^2 + p_j/rho_j^2) gradW_ij
+           + mu m sum_j (v_j - v_i) lapW_ij / rho_j + g rho_i) / rho_i
+    over the pairs with float32 r^2 <= fl(h^2), j != i. Also returns the scale
+    S_i = (m rho_i sum|p-terms| + mu m sum|v-terms| + |g| rho_i) / rho_i per
+    component. `form` plants a mistake: "visc_rho_i" divides the viscosity term
+    by rho_i, "mixed_pressure" uses p_j/rho_i^2 + p_i/rho_j^2."""
+    c = {"spiky": -45.0 / (np.pi * float(H) ** 6), "visc": 45.0 / (np.pi * float(H) ** 6), "h": float(H)}
+    m, mu = float(np.float32(0.02)), float(np.float32(3.5))
+    rho64, p64 = rho.astype(np.float64), prs.astype(np.float64)
+    vel64 = vel.astype(np.float64)
+    g = np.asarray(g, np.float64)
+    acc = np.zeros((len(pos), 3))
+    scale = np.zeros((len(pos), 3))
+    for a in range(0, len(pos), rows):
+        b = min(a + rows, len(pos))
+        d = (pos[a:b, None, :] - pos[None, :, :]).astype(np.float32)
+        r2 = ((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]).astype(np.float32)
+        ok = r2 <= HSQ
+        ok[np.arange(b - a), np.arange(a, b)] = False
+        r = np.sqrt(np.where(ok, r2, 1.0).astype(np.float64))
+        grad = (c["spiky"] * (c["h"] - r) ** 2 / r)[..., None] * d.astype(np.float64)
+        ri, pi = rho64[a:b, None], p64[a:b, None]
+        rj, pj = rho64[None, :], p64[None, :]
+        pf = pj / ri ** 2 + pi / rj ** 2 if form == "mixed_pressure" else pi / ri ** 2 + pj / rj ** 2
+        pt = np.where(ok[..., None], pf[..., None] * grad, 0.0)
+        lap = c["visc"] * (c["h"] - r)
+        div = np.broadcast_to(ri, ok.shape) if form == "visc_rho_i" else np.broadcast_to(rj, ok.shape)
+        vt = np.where(ok[..., None], (vel64[None, :, :] - vel64[a:b, None, :]) * (lap / div)[..., None], 0.0)
+        rr = rho64[a:b, None]
+        acc[a:b] = (-m * rr * pt.sum(1) + mu * m * vt.sum(1) + g[None, :] * rr) / rr
+        scale[a:b] = (m * rr * np.abs(pt).sum(1) + mu * m * np.abs(vt).sum(1) + np.linalg.norm(g) * rr) / rr
+    return acc, scale
+
+
+def test_forces_against_all_pairs_with_unequal_densities(orc):
+    """A cloud with a dense core and a sparse halo (densities spread over a
+    factor > 2) and random velocities, away from the walls: every component of
+    the oracle's acceleration is within 1e-5 * S_i of the float64 all-pairs
+    sum. The same check rejects an oracle that divides the viscosity term by
+    rho_i or mixes p_j/rho_i^2 + p_i/rho_j^2: those forms differ from the
+    right one by more than 100x the tolerance."""
+    rng = np.random.default_rng(21)
+    n = 1600
+    pos = ((rng.random((n, 3)) - 0.5) * np.float32(0.24)).astype(np.float32)
+    pos[:500] = (pos[:500] * np.float32(0.35)).astype(np.float32)   # the dense core
+    vel = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    g = (0.0, -9.80665, 0.0)
+    for mode in (orc.ORDER_REFERENCE, orc.ORDER_CANONICAL):
+        s = orc.OracleSim(0.4, n)
+        s.add_particles(pos, vel)
+        s.set_order_mode(mode)
+        s.update_grid()
+        s.update_density_pressure()
+        s.update_forces()
+        st = s.state()
+        rho, prs = st["density"], st["pressure"]
+        assert rho.max() > 2 * rho.min()
+        want, scale = _brute_accel(pos, vel, rho, prs, g)
+        err = np.abs(st["acc"].astype(np.float64) - want) / scale
+        assert err.max() < 1e-5, (mode, float(err.max()))
+    for form in ("visc_rho_i", "mixed_pressure"):
+        wrong, _ = _brute_accel(pos, vel, rho, prs, g, form=form)
+        assert (np.abs(wrong - want) / scale).max() > 1e-3, form
+
+
 def test_pressure_forces_cancel(orc, scene_4k):
     """With zero velocities the viscosity term vanishes and the pressure forces
     are pairwise antisymmetric: sum_i (a_i - g - wall_i) = 0 up to rounding of
