@@ -99,7 +99,7 @@ def copy_calibration(nbytes):
 def load_traffic(workload):
     """HBM bytes per launch of each kernel from the committed counter summary
     (profiles/traffic_<workload>.json, FETCH_SIZE + WRITE_SIZE scaled by a copy
-    of known size), or None."""
+    of known size, over the steps its "steps" field names), or None."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     try:
         with open(path) as f:
@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="dam_break_1m")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--steady-steps", type=int, default=100,
+                    help="steps timed from step 200 on as well (0 skips; only when the main window ends by 200)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=500, help="0 skips the batched measurement")
     ap.add_argument("--per-scene-steps", type=int, default=None,
@@ -364,12 +366,38 @@ def main():
         if ms > 0:
             gbs = nbytes / (ms * 1e-3) / 1e9
             stage_roofline[name] = {"ms": ms, "alg_bytes": nbytes, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    # the counter summary counts only if it covers the steps timed here
+    timed = [args.warmup, args.warmup + args.steps - 1]
     traffic = load_traffic(args.workload)
+    traffic_ok = bool(traffic) and traffic.get("steps") == timed
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": traffic["kernels"].get(dominant, {}).get("bytes_per_launch") if traffic else None,
-                "avg_launch_ms": dom_ms, "alg_bytes_per_particle": ALG_BYTES[dominant],
+                "traffic": traffic["kernels"].get(dominant, {}).get("bytes_per_launch") if traffic_ok else None,
+                "traffic_steps": traffic.get("steps") if traffic else None,
+                "avg_launch_ms": dom_ms, "timed_steps": "%d-%d" % tuple(timed),
+                "alg_bytes_per_particle": ALG_BYTES[dominant],
                 "step_alg_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9}
+
+    # The early window (the lattice has not collapsed yet) next to a later one:
+    # the same context continued to step 200, then steps 200-299 timed.
+    steady = None
+    if args.steady_steps and args.warmup + args.steps <= 200:
+        prof.advance(200 - (args.warmup + args.steps))
+        prof.stage_times(reset=True)
+        sim.advance(200 - (args.warmup + args.steps))
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.advance(args.steady_steps)
+        sim.sync()
+        sw = time.perf_counter() - t0
+        prof.advance(args.steady_steps)
+        sst = prof.stage_times()
+        sst = {k: (ms / cnt if cnt else 0.0) for k, (ms, cnt) in sst.items()}
+        sdom = max(("density", "forces"), key=lambda k: sst[k])
+        steady = {"timed_steps": "200-%d" % (199 + args.steady_steps), "ms_per_step": sw * 1e3 / args.steady_steps,
+                  "value": n * args.steady_steps / sw, "stage_ms": sst,
+                  "dominant": {"kernel": sdom, "avg_launch_ms": sst[sdom],
+                               "achieved": ALG_BYTES[sdom] * n / (sst[sdom] * 1e-3) / 1e9}}
     prof.close()
 
     line = {
@@ -380,6 +408,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "timed_steps": "%d-%d" % tuple(timed),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -392,6 +421,7 @@ def main():
         "stage_ms": stages,
         "stage_roofline": stage_roofline,
         "roofline": roofline,
+        "steady_state": steady,
     }
     sim.close()
     if args.batched_steps:
