@@ -110,6 +110,16 @@ struct sph_ctx {
     uint32_t *partials = nullptr;
     uint32_t *cstart = nullptr;
 
+    // reference order (SPH_FLAG_REFERENCE_ORDER): each live slot's cell from the
+    // last grid phase (local traversal number), the old cells as global keys, the
+    // grid phase's work buffers and a one-segment tiling of the slots
+    uint32_t *ocell = nullptr, *gold = nullptr;
+    sph::RefBuffers ref{};
+    uint32_t ref_cells_cap = 0;
+    std::vector<sph::SortTile> tiles_flat_h;
+    sph::SortTile *tiles_flat_d = nullptr;
+    int flat_bits = 1;
+
     float *rho = nullptr;
     uint32_t *rec = nullptr;
     float *acc[3] = {};
@@ -176,6 +186,8 @@ struct sph_ctx {
         s.id = ids[idw];
         return s;
     }
+
+    bool ref_order() const { return (flags & SPH_FLAG_REFERENCE_ORDER) != 0; }
 
     sph::SceneTable table() const
     {
@@ -506,6 +518,38 @@ sph_status rebuild_layout(sph_ctx *c)
         SPH_TRY(dalloc(c, &c->cstart, want));
         c->cstart_cap = want;
     }
+    if (c->ref_order()) {
+        if (c->total_cells + 1 > c->ref_cells_cap) {
+            for (uint32_t **a : {&c->ref.n_early, &c->ref.n_arrive, &c->ref.n_depart, &c->ref.nstay, &c->ref.partials}) {
+                dfree(c, *a);
+                *a = nullptr;
+            }
+            const uint32_t want = c->cstart_cap;
+            SPH_TRY(dalloc(c, &c->ref.n_early, want));
+            SPH_TRY(dalloc(c, &c->ref.n_arrive, want));
+            SPH_TRY(dalloc(c, &c->ref.n_depart, want));
+            SPH_TRY(dalloc(c, &c->ref.nstay, want));
+            SPH_TRY(dalloc(c, &c->ref.partials, sph::scan_blocks(want)));
+            c->ref_cells_cap = want;
+        }
+        // one segment over the active slots, for the sort of the movers by global destination
+        c->tiles_flat_h.clear();
+        const uint32_t nt = (c->nslots + sph::kSortTile - 1) / sph::kSortTile;
+        for (uint32_t k = 0; k < nt; ++k) {
+            sph::SortTile tl{};
+            tl.start = k * (uint32_t) sph::kSortTile;
+            tl.count = std::min<uint32_t>((uint32_t) sph::kSortTile, c->nslots - tl.start);
+            tl.coff = 0;
+            tl.hbase = k;
+            tl.stride = nt;
+            c->tiles_flat_h.push_back(tl);
+        }
+        if (nt > c->tiles_cap) return fail(SPH_ERR_CAPACITY, "sort tiles exceed their buffer");
+        if (nt)
+            SPH_HIP(hipMemcpy(c->tiles_flat_d, c->tiles_flat_h.data(), nt * sizeof(sph::SortTile),
+                              hipMemcpyHostToDevice));
+        c->flat_bits = key_bits(c->total_cells + 1);
+    }
     SPH_HIP(hipMemcpy(c->table_d, c->table_h.data(), c->table_h.size() * sizeof(sph::Scene), hipMemcpyHostToDevice));
     if (!c->group_h.empty())
         SPH_HIP(hipMemcpy(c->group_d, c->group_h.data(), c->group_h.size() * sizeof(uint32_t),
@@ -583,14 +627,63 @@ bool find_range(const sph_ctx *c, uint32_t cap, uint32_t &poff, uint32_t &seg)
 sph_status run_emit(sph_ctx *c)
 {
     if (c->emit_h.empty()) return SPH_OK;
-    sph::launch_emit(c->emit_d, (uint32_t) c->emit_h.size(), c->table_d, c->soa(c->cur, c->id_cur), c->key_h, c->k.h,
-                     c->stream);
+    sph::launch_emit(c->emit_d, (uint32_t) c->emit_h.size(), c->table_d, c->soa(c->cur, c->id_cur), c->key_h,
+                     c->ref_order() ? c->ocell : nullptr, c->k.h, c->stream);
     c->counts_stale = true;
     return launch_error("emitter launch");
 }
 
+// Stable LSD radix sort of (keys, vals) over the given tiles into key_s / val_s
+// (vals == nullptr: the slot index), in passes of 8 bits up to `bits`.
+void radix_sort(sph_ctx *c, const uint32_t *keys, const uint32_t *vals, const sph::SortTile *tiles, uint32_t ntiles,
+                int bits)
+{
+    const int passes = (bits + sph::kRadixBits - 1) / sph::kRadixBits;
+    const uint32_t *kin = keys;
+    const uint32_t *vin = vals;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_s = ((passes - 1 - p) % 2) == 0;
+        uint32_t *kout = to_s ? c->key_s : c->key_t;
+        uint32_t *vout = to_s ? c->val_s : c->val_t;
+        sph::launch_radix_pass(kin, vin, kout, vout, tiles, ntiles, p * sph::kRadixBits, c->hist, c->partials,
+                               c->stream);
+        kin = kout;
+        vin = vout;
+    }
+}
+
+// Grid phase in the reference's bucket order (see the reference-order kernels):
+// the old buckets (a stable sort by old cell, so that particles added since the
+// last grid phase follow cell (0,0,0)'s bucket), per-cell counts, the per-cell
+// swap-remove scans, the movers sorted by destination in push order, the new
+// cell table, then the gather.
+sph_status run_grid_reference(sph_ctx *c)
+{
+    const uint32_t n = c->nslots, C = c->total_cells;
+    const sph::SceneTable tab = c->table();
+    {
+        StageTimer t(c, SPH_STAGE_SORT);
+        if (t.st) return t.st;
+        sph::launch_ref_prepare(c->ocell, c->gold, tab, c->stream);
+        radix_sort(c, c->gold, nullptr, c->tiles_d, (uint32_t) c->tiles_h.size(), c->sort_bits);
+        sph::launch_cell_start(c->key_s, n, C, c->cstart, c->stream);
+        sph::launch_ref_counts(c->ref, c->gold, c->key_h, n, C, c->stream);
+        sph::launch_ref_bucket(c->ref, C, c->cstart, c->val_s, c->key_h, c->stream);
+        sph::launch_ref_movers(c->ref, c->gold, c->key_h, n, C, c->stream);
+        radix_sort(c, c->ref.mkey, c->ref.mval, c->tiles_flat_d, (uint32_t) c->tiles_flat_h.size(), c->flat_bits);
+    }
+    {
+        StageTimer t(c, SPH_STAGE_CELLS);
+        if (t.st) return t.st;
+        sph::launch_ref_place(c->ref, c->key_s, c->val_s, n, C, c->cstart, c->stream);
+        sph::launch_ref_final(c->ref, c->key_h, c->cstart, c->key_s, c->val_s, c->ocell, tab, c->stream);
+    }
+    return SPH_OK;
+}
+
 // Grid phase: hash (unless the previous integrate already did), stable LSD radix
-// sort of (cell, slot), cell start table, and gather into sorted order.
+// sort of (cell, slot) -- or the reference's buckets -- cell start table, and
+// gather into sorted order.
 sph_status run_grid(sph_ctx *c)
 {
     const uint32_t n = c->nslots;
@@ -601,23 +694,14 @@ sph_status run_grid(sph_ctx *c)
         if (t.st) return t.st;
         sph::launch_hash(cur, c->key_h, tab, c->k.h, c->stream);
     }
-    {
-        StageTimer t(c, SPH_STAGE_SORT);
-        if (t.st) return t.st;
-        const int passes = (c->sort_bits + sph::kRadixBits - 1) / sph::kRadixBits;
-        const uint32_t *kin = c->key_h;
-        const uint32_t *vin = nullptr;
-        for (int p = 0; p < passes; ++p) {
-            const bool to_s = ((passes - 1 - p) % 2) == 0;
-            uint32_t *kout = to_s ? c->key_s : c->key_t;
-            uint32_t *vout = to_s ? c->val_s : c->val_t;
-            sph::launch_radix_pass(kin, vin, kout, vout, c->tiles_d, (uint32_t) c->tiles_h.size(), p * sph::kRadixBits,
-                                   c->hist, c->partials, c->stream);
-            kin = kout;
-            vin = vout;
+    if (c->ref_order()) {
+        SPH_TRY(run_grid_reference(c));
+    } else {
+        {
+            StageTimer t(c, SPH_STAGE_SORT);
+            if (t.st) return t.st;
+            radix_sort(c, c->key_h, nullptr, c->tiles_d, (uint32_t) c->tiles_h.size(), c->sort_bits);
         }
-    }
-    {
         StageTimer t(c, SPH_STAGE_CELLS);
         if (t.st) return t.st;
         sph::launch_cell_start(c->key_s, n, c->total_cells, c->cstart, c->stream);
@@ -807,7 +891,13 @@ sph_status write_slots(sph_ctx *c, const float *pos3, const float *vel3, const u
         SPH_TRY(copy_h2d(c, c->vel[c->cur][comp] + first, buf.data(), n * sizeof(float)));
     }
     for (uint32_t i = 0; i < n; ++i) ib[i] = ids ? ids[i] : id_base + i;
-    return copy_h2d(c, c->ids[c->id_cur] + first, ib.data(), n * sizeof(uint32_t));
+    SPH_TRY(copy_h2d(c, c->ids[c->id_cur] + first, ib.data(), n * sizeof(uint32_t)));
+    if (c->ref_order()) {
+        // uploaded and appended particles join the back of cell (0,0,0)'s bucket (addParticle)
+        std::fill(ib.begin(), ib.end(), 0u);
+        SPH_TRY(copy_h2d(c, c->ocell + first, ib.data(), n * sizeof(uint32_t)));
+    }
+    return SPH_OK;
 }
 
 sph_status scene_write(sph_ctx *c, uint32_t scene, const float *pos3, const float *vel3, const uint32_t *ids,
@@ -966,6 +1056,16 @@ sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
     SPH_TRY(dalloc(c, &c->hist, hist_n));
     SPH_TRY(dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n)));
     SPH_TRY(dalloc(c, &c->tiles_d, c->tiles_cap));
+    if (c->ref_order()) {
+        SPH_TRY(dalloc(c, &c->ocell, P));
+        SPH_TRY(dalloc(c, &c->gold, P));
+        SPH_TRY(dalloc(c, &c->ref.rk, P));
+        SPH_TRY(dalloc(c, &c->ref.epos, P));
+        SPH_TRY(dalloc(c, &c->ref.mkey, P));
+        SPH_TRY(dalloc(c, &c->ref.mval, P));
+        SPH_TRY(dalloc(c, &c->ref.work, 2 * P));
+        SPH_TRY(dalloc(c, &c->tiles_flat_d, c->tiles_cap));
+    }
     SPH_TRY(dalloc(c, &c->rho, P));
     SPH_TRY(dalloc(c, &c->rec, (size_t) sph::kRecWords * P));
     for (int a = 0; a < 3; ++a) SPH_TRY(dalloc(c, &c->acc[a], P));
@@ -1031,6 +1131,7 @@ sph_status create_ctx(uint32_t pool, uint32_t max_scenes, uint32_t flags, uint32
         c->scenes[i].used = true;
         c->scenes[i].req_cap = descs[i].capacity;
         c->scenes[i].s = placed[i];
+        c->scenes[i].s.order = c->ref_order() ? sph::kOrderReference : sph::kOrderCanonical;
     }
     if (const char *f = std::getenv("SPH_FAULT_AT_STEP")) c->fault_at_step = std::atol(f);
 
@@ -1065,7 +1166,7 @@ struct Snapshot {
     {
         const size_t P = c->pool;
         sph_status st;
-        mem = dalloc_raw(c, P * 32, &st);
+        mem = dalloc_raw(c, P * 36, &st);
         if (st) return st;
         char *m = (char *) mem;
         for (int a = 0; a < 3; ++a) {
@@ -1074,6 +1175,7 @@ struct Snapshot {
         }
         SPH_HIP(hipMemcpyAsync(m + 6 * P * 4, c->ids[c->id_cur], P * 4, hipMemcpyDeviceToDevice, c->stream));
         SPH_HIP(hipMemcpyAsync(m + 7 * P * 4, c->key_h, P * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (c->ocell) SPH_HIP(hipMemcpyAsync(m + 8 * P * 4, c->ocell, P * 4, hipMemcpyDeviceToDevice, c->stream));
         SPH_HIP(hipStreamSynchronize(c->stream));
         table = c->table_h;
         SPH_HIP(hipMemcpy(table.data(), c->table_d, table.size() * sizeof(sph::Scene), hipMemcpyDeviceToHost));
@@ -1097,6 +1199,7 @@ struct Snapshot {
         }
         SPH_HIP(hipMemcpyAsync(c->ids[id_cur], m + 6 * P * 4, P * 4, hipMemcpyDeviceToDevice, c->stream));
         SPH_HIP(hipMemcpyAsync(c->key_h, m + 7 * P * 4, P * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (c->ocell) SPH_HIP(hipMemcpyAsync(c->ocell, m + 8 * P * 4, P * 4, hipMemcpyDeviceToDevice, c->stream));
         SPH_HIP(hipMemcpyAsync(c->table_d, table.data(), table.size() * sizeof(sph::Scene), hipMemcpyHostToDevice,
                                c->stream));
         SPH_HIP(hipStreamSynchronize(c->stream));
@@ -1211,6 +1314,7 @@ sph_status sph_scene_add(sph_ctx *c, const sph_scene_desc *desc, uint32_t *scene
                                               " slots in the pool");
         s.poff = poff;
         s.cap = seg;
+        s.order = c->ref_order() ? sph::kOrderReference : sph::kOrderCanonical;
         SceneRec &r = c->scenes[id];
         r = SceneRec{};
         r.used = true;

@@ -43,8 +43,20 @@ struct Consts {
 // scenes from a table in device memory, so live counts can change on the device
 // and a captured step stays valid. A free range of slots is a scene with no cells
 // and no particles.
+// Cell numbering and summation order of a scene.
+enum SceneOrder : uint32_t {
+    // cells x + y*rx + z*rx*ry; a cell's particles in the stable order of the
+    // previous array order; neighbour rows z (outer), y, then the run of x cells
+    kOrderCanonical = 0,
+    // the reference's own order (src/CCPUParticleSimulator.cpp:37-88, 108-121):
+    // cells numbered in its traversal order T = z + y*rz + x*rz*ry, a cell's
+    // particles in its swap-remove bucket order, neighbour rows x (outer), y,
+    // then the run of z cells (contiguous in T)
+    kOrderReference = 1,
+};
+
 struct Scene {
-    int rx, ry, rz;        // grid resolution (CGrid), cells x + y*rx + z*rx*ry
+    int rx, ry, rz;        // grid resolution (CGrid)
     uint32_t ncells;       // rx * ry * rz
     uint32_t coff;         // first global cell id of the scene
     uint32_t poff;         // first particle slot of the scene
@@ -53,7 +65,7 @@ struct Scene {
     float bx, by, bz;      // box extents (walls at +-b/2)
     float gx, gy, gz;      // gravity
     float dt;
-    uint32_t pad;
+    uint32_t order;        // SceneOrder
 };
 static_assert(sizeof(Scene) == 64, "Scene is read as 16 dwords");
 

@@ -91,8 +91,8 @@ extern const NeighbourVariant kNeighbourVariants[kNumVariants];
 
 // Emitters (one workgroup each) add particles at the start of a step and update
 // the live counts in the device scene table.
-void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleSoA &p, uint32_t *key, double h,
-                 hipStream_t st);
+void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleSoA &p, uint32_t *key, uint32_t *ocell,
+                 double h, hipStream_t st);
 
 // Cell key of every slot below t.nslots: the hash of src/CCPUParticleSimulator.cpp:46-69
 // for live particles, the scene's dead cell for unused slots.
@@ -120,6 +120,35 @@ inline uint32_t max_sort_tiles(uint32_t pool, uint32_t max_segments) { return so
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
                        uint32_t *vals_out, const SortTile *tiles, uint32_t ntiles, int shift, uint32_t *hist,
                        uint32_t *partials, hipStream_t st);
+
+// In-place exclusive scan of m values; partials holds scan_blocks(m) entries.
+void launch_exclusive_scan(uint32_t *data, uint32_t m, uint32_t *partials, hipStream_t st);
+
+// Buffers of the reference-order grid phase (per slot: rk, epos, mkey, mval; 2 per
+// slot: work; per cell + 1: n_early, n_arrive, n_depart, nstay; partials for scans
+// of cell arrays).
+struct RefBuffers {
+    uint32_t *rk, *epos, *mkey, *mval, *work;
+    uint32_t *n_early, *n_arrive, *n_depart, *nstay;
+    uint32_t *partials;
+};
+
+// The reference-order grid phase in steps (see sph_kernels.hip), over n slots and
+// ncells global cells: old cells as keys (gold), the counts and their scans, the
+// per-cell swap-remove scans over the old buckets (sorted by old cell: ostart,
+// perm), the movers in push order (keys mkey, values mval, to be sorted by key),
+// their placing and the new cell table (start), the permutation and keys.
+void launch_ref_prepare(const uint32_t *ocell, uint32_t *gold, const SceneTable &t, hipStream_t st);
+void launch_ref_counts(const RefBuffers &r, const uint32_t *gold, const uint32_t *gnew, uint32_t n, uint32_t ncells,
+                       hipStream_t st);
+void launch_ref_bucket(const RefBuffers &r, uint32_t ncells, const uint32_t *ostart, const uint32_t *perm,
+                       const uint32_t *gnew, hipStream_t st);
+void launch_ref_movers(const RefBuffers &r, const uint32_t *gold, const uint32_t *gnew, uint32_t n, uint32_t ncells,
+                       hipStream_t st);
+void launch_ref_place(const RefBuffers &r, const uint32_t *skey, const uint32_t *sval, uint32_t n, uint32_t ncells,
+                      uint32_t *start, hipStream_t st);
+void launch_ref_final(const RefBuffers &r, const uint32_t *gnew, const uint32_t *start, uint32_t *skey, uint32_t *perm,
+                      uint32_t *ocell, const SceneTable &t, hipStream_t st);
 
 // start[c] = first slot whose sorted key is >= c, c in [0, ncells].
 void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells, uint32_t *start,

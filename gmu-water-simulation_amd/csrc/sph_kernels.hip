@@ -37,14 +37,22 @@ __device__ __forceinline__ int clamp_cell(int c, int r)
     return c < 0 ? 0 : (c >= r ? r - 1 : c);
 }
 
-// Cell hash, src/CCPUParticleSimulator.cpp:46-69 with include/CGrid.h:25-28:
-// floor(((double)p + (double)box/2.0) / (double)h), clamped, linearised x-fastest.
+// Local cell number of grid cell (x, y, z): x + y*rx + z*rx*ry (include/CGrid.h:25-28)
+// in the canonical order, z + y*rz + x*rz*ry (the reference's traversal order,
+// src/CCPUParticleSimulator.cpp:37-39) in the reference order.
+__device__ __forceinline__ uint32_t cell_number(int x, int y, int z, const Scene &s)
+{
+    return (uint32_t) (s.order == kOrderReference ? z + y * s.rz + x * s.rz * s.ry : x + y * s.rx + z * s.rx * s.ry);
+}
+
+// Cell hash, src/CCPUParticleSimulator.cpp:46-69:
+// floor(((double)p + (double)box/2.0) / (double)h), clamped, numbered as cell_number.
 __device__ __forceinline__ uint32_t hash_key(float px, float py, float pz, const Scene &s, double h)
 {
     const int x = clamp_cell(to_int_x86(floor(((double) px + (double) s.bx / 2.0) / h)), s.rx);
     const int y = clamp_cell(to_int_x86(floor(((double) py + (double) s.by / 2.0) / h)), s.ry);
     const int z = clamp_cell(to_int_x86(floor(((double) pz + (double) s.bz / 2.0) / h)), s.rz);
-    return s.coff + (uint32_t) (x + y * s.rx + z * s.rx * s.ry);
+    return s.coff + cell_number(x, y, z, s);
 }
 
 // CCollisionGeometry::inverseBoundingBoxBounce (src/CCollisionGeometry.cpp:117-132)
@@ -110,10 +118,17 @@ __device__ __forceinline__ uint32_t dead_key(const Scene &s)
 __device__ __forceinline__ void cell_xyz(uint32_t key, const Scene &s, int &x, int &y, int &z)
 {
     const uint32_t c = key - s.coff;
-    x = (int) (c % (uint32_t) s.rx);
-    const uint32_t rem = c / (uint32_t) s.rx;
-    y = (int) (rem % (uint32_t) s.ry);
-    z = (int) (rem / (uint32_t) s.ry);
+    if (s.order == kOrderReference) {
+        z = (int) (c % (uint32_t) s.rz);
+        const uint32_t rem = c / (uint32_t) s.rz;
+        y = (int) (rem % (uint32_t) s.ry);
+        x = (int) (rem / (uint32_t) s.ry);
+    } else {
+        x = (int) (c % (uint32_t) s.rx);
+        const uint32_t rem = c / (uint32_t) s.rx;
+        y = (int) (rem % (uint32_t) s.ry);
+        z = (int) (rem / (uint32_t) s.ry);
+    }
 }
 
 // ---------------------------------------------------------------- grid
@@ -151,7 +166,7 @@ __device__ __forceinline__ float unit24(uint64_t x)
 // centre), then v_y = speed * (1 + spread*(2u-1)) from draw 31. Every operation
 // is a single IEEE float operation, so a CPU restatement gives the same bits.
 __global__ __launch_bounds__(256) void k_emit(const Emitter *__restrict__ em, Scene *scenes, ParticleSoA p,
-                                              uint32_t *__restrict__ key, double h)
+                                              uint32_t *__restrict__ key, uint32_t *__restrict__ ocell, double h)
 {
     __shared__ uint32_t sh[2];
     const Emitter e = em[blockIdx.x];
@@ -204,6 +219,7 @@ __global__ __launch_bounds__(256) void k_emit(const Emitter *__restrict__ em, Sc
         p.vz[slot] = 0.0f;
         p.id[slot] = g;
         key[slot] = hash_key(x, floor_y, z, s, h);
+        if (ocell) ocell[slot] = 0u;   // addParticle puts it at the end of cell (0,0,0)'s bucket
     }
     __syncthreads();
     if (threadIdx.x == 0) sp->n = n0 + cnt;
@@ -499,6 +515,179 @@ __global__ __launch_bounds__(256) void k_scatter_by_id(ParticleSoA p, uint32_t f
     }
 }
 
+// ---------------------------------------------------------------- reference bucket order
+//
+// CCPUParticleSimulator::updateGrid (src/CCPUParticleSimulator.cpp:37-88) keeps
+// each cell's particles in a bucket and visits the cells in its traversal order
+// T (x outer, z inner). In a visited bucket a particle whose cell changed is
+// pushed to the back of its new cell's bucket and replaced by the bucket's last
+// element (swap-remove), which is then examined in its place. New particles are
+// pushed to the back of cell (0,0,0)'s bucket (src/CBaseParticleSimulator.cpp:67-74).
+// Cells are numbered by T (cell_number), so the array order -- cells in T order,
+// each cell's bucket in order -- is the reference's whole state.
+//
+// The kernels below compute the buckets after one updateGrid in parallel:
+//  * The bucket a cell holds when it is visited is its old bucket followed by
+//    its early arrivals: movers from cells visited before it (smaller T), in
+//    push order. Early arrivals never move again, so the swap-remove scan of a
+//    cell depends on them only through their count, which is order-free.
+//  * Per cell, one thread runs the scan on (old bucket, early-arrival
+//    placeholders): the final position of every stayer and placeholder, and the
+//    departure rank of every mover.
+//  * Pushes happen in order of (source T, departure rank). Movers sorted by
+//    (destination, that order) give each cell its arrivals: the first n_early
+//    fill the placeholders, the rest (from later cells, pushed after the scan)
+//    go to the back in order.
+// Every slot of the pool takes part (dead and free slots in their segment's
+// dead cell, where they never move). ocell holds each live slot's cell from the
+// previous grid phase as a local T number (0 for particles added since).
+
+// Old cell of every slot as a global key (the dead cell for unused slots).
+__global__ __launch_bounds__(256) void k_ref_prepare(const uint32_t *__restrict__ ocell, uint32_t *__restrict__ gold,
+                                                     SceneTable t)
+{
+    const uint32_t first = blockIdx.x * 256u, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    if (i >= t.nslots) return;
+    gold[i] = (i - s.poff < s.n) ? s.coff + ocell[i] : dead_key(s);
+}
+
+__global__ __launch_bounds__(256) void k_ref_zero(uint32_t *__restrict__ a, uint32_t *__restrict__ b,
+                                                  uint32_t *__restrict__ c, uint32_t *__restrict__ d, uint32_t m)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= m) return;
+    a[i] = 0u;
+    b[i] = 0u;
+    c[i] = 0u;
+    d[i] = 0u;
+}
+
+// Per cell: arrivals, departures and early arrivals (movers from smaller T).
+__global__ __launch_bounds__(256) void k_ref_count(const uint32_t *__restrict__ gold, const uint32_t *__restrict__ gnew,
+                                                   uint32_t n, uint32_t *__restrict__ n_early,
+                                                   uint32_t *__restrict__ n_arrive, uint32_t *__restrict__ n_depart)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = gold[i], c = gnew[i];
+    if (o == c) return;
+    atomicAdd(&n_arrive[c], 1u);
+    atomicAdd(&n_depart[o], 1u);
+    if (o < c) atomicAdd(&n_early[c], 1u);
+}
+
+constexpr uint32_t kPlaceholder = 0x80000000u;
+
+// The swap-remove scan of cell c (one thread per cell) over its old bucket
+// (slots perm[ostart[c] .. ostart[c+1]), in order) followed by its early-arrival
+// placeholders, in work[ostart[c] + eoff[c] ...]. rk gets each stayer's final
+// position and each mover's departure rank; epos[eoff[c] + k] the final position
+// of placeholder k; nstay[c] the bucket's length after the scan.
+__global__ __launch_bounds__(256) void k_ref_bucket(uint32_t ncells, const uint32_t *__restrict__ ostart,
+                                                    const uint32_t *__restrict__ perm,
+                                                    const uint32_t *__restrict__ gnew,
+                                                    const uint32_t *__restrict__ eoff, uint32_t *__restrict__ work,
+                                                    uint32_t *__restrict__ rk, uint32_t *__restrict__ epos,
+                                                    uint32_t *__restrict__ nstay)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= ncells) return;
+    const uint32_t a = ostart[c], n = ostart[c + 1] - a;
+    const uint32_t e0 = eoff[c], ne = eoff[c + 1] - e0;
+    uint32_t *w = work + a + e0;
+    for (uint32_t q = 0; q < n; ++q) w[q] = perm[a + q];
+    for (uint32_t k = 0; k < ne; ++k) w[n + k] = kPlaceholder | k;
+    uint32_t len = n + ne, p = 0, dep = 0;
+    while (p < len) {
+        const uint32_t e = w[p];
+        if (!(e & kPlaceholder) && gnew[e] != c) {
+            rk[e] = dep++;
+            w[p] = w[len - 1];
+            --len;
+        } else {
+            ++p;
+        }
+    }
+    for (uint32_t q = 0; q < len; ++q) {
+        const uint32_t e = w[q];
+        if (e & kPlaceholder)
+            epos[e0 + (e & ~kPlaceholder)] = q;
+        else
+            rk[e] = q;
+    }
+    nstay[c] = len;
+}
+
+// Movers in push order: mover i goes to position moff[old cell] + departure rank,
+// with its destination as key. Positions from the total mover count on are
+// padding (key ncells, after every cell).
+__global__ __launch_bounds__(256) void k_ref_movers(const uint32_t *__restrict__ gold, const uint32_t *__restrict__ gnew,
+                                                    const uint32_t *__restrict__ rk, const uint32_t *__restrict__ moff,
+                                                    uint32_t n, uint32_t ncells, uint32_t *__restrict__ mkey,
+                                                    uint32_t *__restrict__ mval)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = gold[i], c = gnew[i];
+    if (o != c) {
+        const uint32_t d = moff[o] + rk[i];   // < the mover count <= n by construction
+        if (d < n) {
+            mkey[d] = c;
+            mval[d] = i;
+        }
+    }
+    if (i >= moff[ncells]) {
+        mkey[i] = ncells;
+        mval[i] = i;
+    }
+}
+
+// Position q of the movers sorted by destination (push order within one):
+// arrival k = q - astart[c] of cell c fills placeholder k if it is an early
+// arrival, else goes to the back of the bucket.
+__global__ __launch_bounds__(256) void k_ref_place(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sval,
+                                                   const uint32_t *__restrict__ astart, const uint32_t *__restrict__ eoff,
+                                                   const uint32_t *__restrict__ epos, const uint32_t *__restrict__ nstay,
+                                                   uint32_t n, uint32_t ncells, uint32_t *__restrict__ rk)
+{
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t c = skey[q];
+    if (c >= ncells) return;
+    const uint32_t k = q - astart[c], e0 = eoff[c], ne = eoff[c + 1] - e0;
+    rk[sval[q]] = k < ne ? epos[e0 + k] : nstay[c] + (k - ne);
+}
+
+// Bucket lengths after updateGrid: the scanned bucket plus the late arrivals.
+__global__ __launch_bounds__(256) void k_ref_counts(const uint32_t *__restrict__ nstay,
+                                                    const uint32_t *__restrict__ n_arrive,
+                                                    const uint32_t *__restrict__ eoff, uint32_t ncells,
+                                                    uint32_t *__restrict__ count)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c > ncells) return;
+    count[c] = c < ncells ? nstay[c] + n_arrive[c] - (eoff[c + 1] - eoff[c]) : 0u;
+}
+
+// Slot of every particle in the new array order: its cell's start plus its
+// bucket position. Writes the permutation, the sorted keys and the cells the
+// next grid phase starts from.
+__global__ __launch_bounds__(256) void k_ref_final(const uint32_t *__restrict__ gnew, const uint32_t *__restrict__ rk,
+                                                   const uint32_t *__restrict__ start, uint32_t *__restrict__ skey,
+                                                   uint32_t *__restrict__ perm, uint32_t *__restrict__ ocell,
+                                                   SceneTable t)
+{
+    const uint32_t first = blockIdx.x * 256u, i = first + threadIdx.x;
+    const Scene s = block_scene(t, first);
+    if (i >= t.nslots) return;
+    const uint32_t c = gnew[i], f = start[c] + rk[i];   // < nslots by construction
+    if (f >= t.nslots) return;
+    skey[f] = c;
+    perm[f] = i;
+    ocell[f] = c - s.coff;
+}
+
 // ---------------------------------------------------------------- neighbours
 //
 // Canonical neighbour order: rows z-1..z+1 (outer), y-1..y+1, and within a row
@@ -575,13 +764,16 @@ __device__ __forceinline__ FaceBounds face_bounds(float p, float box, int c, dou
     return b;
 }
 
-// Row r (0..8) of a particle's neighbourhood: z-1..z+1 (outer), y-1..y+1, and in
-// it the contiguous run of cells x-1..x+1 in slot order. Rows and end cells of a
-// run whose distance lower bound exceeds h (with a 1e-5 relative margin, far
-// above the float rounding of r^2) are left out: every candidate they hold would
-// fail the r^2 <= h^2 test, so the accepted neighbours and their order are
-// unchanged. The density and force passes compute the same runs from the same
-// inputs.
+// Row r (0..8) of a particle's neighbourhood. Canonical order: z-1..z+1 (outer),
+// y-1..y+1, and in it the contiguous run of cells x-1..x+1 in slot order.
+// Reference order: x-1..x+1 (outer), y-1..y+1, and the run of cells z-1..z+1,
+// contiguous in the traversal numbering -- the offsetX, offsetY, offsetZ loops of
+// src/CCPUParticleSimulator.cpp:108-121 with each cell's bucket in slot order.
+// Rows and end cells of a run whose distance lower bound exceeds h (with a 1e-5
+// relative margin, far above the float rounding of r^2) are left out: every
+// candidate they hold would fail the r^2 <= h^2 test, so the accepted neighbours
+// and their order are unchanged. The density and force passes compute the same
+// runs from the same inputs.
 struct RowBounds {
     FaceBounds bx, by, bz;
     double lim;
@@ -603,12 +795,24 @@ __device__ __forceinline__ RowBounds row_bounds(const Scene &s, const Consts &k,
 __device__ __forceinline__ void row_run(const RowBounds &rb, const Scene &s, const uint32_t *__restrict__ start, int x,
                                         int y, int z, int r, uint32_t &j, uint32_t &jend)
 {
-    const int oz = r / 3 - 1, oy = r % 3 - 1;
-    const int zz = z + oz, yy = y + oy;
-    const double lyz =
-        (oy < 0 ? rb.by.lo2 : oy > 0 ? rb.by.hi2 : 0.0) + (oz < 0 ? rb.bz.lo2 : oz > 0 ? rb.bz.hi2 : 0.0);
     j = 0;
     jend = 0;
+    const int oy = r % 3 - 1, yy = y + oy;
+    const double ly = oy < 0 ? rb.by.lo2 : oy > 0 ? rb.by.hi2 : 0.0;
+    if (s.order == kOrderReference) {
+        const int ox = r / 3 - 1, xx = x + ox;
+        const double lxy = ly + (ox < 0 ? rb.bx.lo2 : ox > 0 ? rb.bx.hi2 : 0.0);
+        if (xx >= 0 && xx < s.rx && yy >= 0 && yy < s.ry && lxy <= rb.lim) {
+            const uint32_t row = s.coff + (uint32_t) (yy * s.rz + xx * s.rz * s.ry);
+            const int z0 = (z > 0 && lxy + rb.bz.lo2 <= rb.lim) ? z - 1 : z;
+            const int z1 = (z + 1 < s.rz && lxy + rb.bz.hi2 <= rb.lim) ? z + 1 : z;
+            j = start[row + z0] - s.poff;
+            jend = start[row + z1 + 1] - s.poff;
+        }
+        return;
+    }
+    const int oz = r / 3 - 1, zz = z + oz;
+    const double lyz = ly + (oz < 0 ? rb.bz.lo2 : oz > 0 ? rb.bz.hi2 : 0.0);
     if (zz >= 0 && zz < s.rz && yy >= 0 && yy < s.ry && lyz <= rb.lim) {
         const uint32_t row = s.coff + (uint32_t) (yy * s.rx + zz * s.rx * s.ry);
         const int x0 = (x > 0 && lyz + rb.bx.lo2 <= rb.lim) ? x - 1 : x;
@@ -1485,11 +1689,11 @@ const NeighbourVariant kNeighbourVariants[kNumVariants] = {SPH_VARIANTS(SPH_VARI
 
 // ---------------------------------------------------------------- launchers
 
-void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleSoA &p, uint32_t *key, double h,
-                 hipStream_t st)
+void launch_emit(const Emitter *em, uint32_t nem, Scene *scenes, const ParticleSoA &p, uint32_t *key, uint32_t *ocell,
+                 double h, hipStream_t st)
 {
     if (nem == 0) return;
-    hipLaunchKernelGGL(k_emit, dim3(nem), dim3(256), 0, st, em, scenes, p, key, h);
+    hipLaunchKernelGGL(k_emit, dim3(nem), dim3(256), 0, st, em, scenes, p, key, ocell, h);
 }
 
 void launch_scatter_by_id(const ParticleSoA &p, uint32_t first, uint32_t n, float *pos, float *vel, uint32_t *bad,
@@ -1521,6 +1725,70 @@ void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_
     }
     hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in, keys_out, vals_out,
                        tiles, shift, (const uint32_t *) hist);
+}
+
+void launch_exclusive_scan(uint32_t *data, uint32_t m, uint32_t *partials, hipStream_t st)
+{
+    if (m == 0) return;
+    const uint32_t sb = scan_blocks(m);
+    if (sb == 1) {
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, data, m);
+    } else {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(1024), 0, st, (const uint32_t *) data, m, partials);
+        hipLaunchKernelGGL(k_scan_apply, dim3(sb), dim3(1024), 0, st, data, m, (const uint32_t *) partials);
+    }
+}
+
+void launch_ref_prepare(const uint32_t *ocell, uint32_t *gold, const SceneTable &t, hipStream_t st)
+{
+    if (t.nslots == 0) return;
+    hipLaunchKernelGGL(k_ref_prepare, dim3(blocks_for(t.nslots, 256)), dim3(256), 0, st, ocell, gold, t);
+}
+
+void launch_ref_counts(const RefBuffers &r, const uint32_t *gold, const uint32_t *gnew, uint32_t n, uint32_t ncells,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ref_zero, dim3(blocks_for(ncells + 1, 256)), dim3(256), 0, st, r.n_early, r.n_arrive,
+                       r.n_depart, r.nstay, ncells + 1);
+    if (n) hipLaunchKernelGGL(k_ref_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, gold, gnew, n, r.n_early,
+                              r.n_arrive, r.n_depart);
+    launch_exclusive_scan(r.n_early, ncells + 1, r.partials, st);
+    launch_exclusive_scan(r.n_depart, ncells + 1, r.partials, st);
+}
+
+void launch_ref_bucket(const RefBuffers &r, uint32_t ncells, const uint32_t *ostart, const uint32_t *perm,
+                       const uint32_t *gnew, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ref_bucket, dim3(blocks_for(ncells, 256)), dim3(256), 0, st, ncells, ostart, perm, gnew,
+                       (const uint32_t *) r.n_early, r.work, r.rk, r.epos, r.nstay);
+}
+
+void launch_ref_movers(const RefBuffers &r, const uint32_t *gold, const uint32_t *gnew, uint32_t n, uint32_t ncells,
+                       hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_ref_movers, dim3(blocks_for(n, 256)), dim3(256), 0, st, gold, gnew,
+                              (const uint32_t *) r.rk, (const uint32_t *) r.n_depart, n, ncells, r.mkey, r.mval);
+}
+
+void launch_ref_place(const RefBuffers &r, const uint32_t *skey, const uint32_t *sval, uint32_t n, uint32_t ncells,
+                      uint32_t *start, hipStream_t st)
+{
+    // r.n_depart (scanned departures) is free again: it holds the arrival starts
+    launch_cell_start(skey, n, ncells, r.n_depart, st);
+    if (n) hipLaunchKernelGGL(k_ref_place, dim3(blocks_for(n, 256)), dim3(256), 0, st, skey, sval,
+                              (const uint32_t *) r.n_depart, (const uint32_t *) r.n_early, (const uint32_t *) r.epos,
+                              (const uint32_t *) r.nstay, n, ncells, r.rk);
+    hipLaunchKernelGGL(k_ref_counts, dim3(blocks_for(ncells + 1, 256)), dim3(256), 0, st, (const uint32_t *) r.nstay,
+                       (const uint32_t *) r.n_arrive, (const uint32_t *) r.n_early, ncells, start);
+    launch_exclusive_scan(start, ncells + 1, r.partials, st);
+}
+
+void launch_ref_final(const RefBuffers &r, const uint32_t *gnew, const uint32_t *start, uint32_t *skey, uint32_t *perm,
+                      uint32_t *ocell, const SceneTable &t, hipStream_t st)
+{
+    if (t.nslots == 0) return;
+    hipLaunchKernelGGL(k_ref_final, dim3(blocks_for(t.nslots, 256)), dim3(256), 0, st, gnew, (const uint32_t *) r.rk,
+                       start, skey, perm, ocell, t);
 }
 
 void launch_cell_start(const uint32_t *sorted_keys, uint32_t n, uint32_t ncells, uint32_t *start, hipStream_t st)

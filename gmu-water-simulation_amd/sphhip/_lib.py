@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsph_hip.so")
 SPH_OK, SPH_ERR_INVALID, SPH_ERR_CAPACITY, SPH_ERR_HIP, SPH_ERR_OOM, SPH_ERR_NO_DEVICE, SPH_ERR_STATE, \
     SPH_ERR_FAULT = range(8)
 PHASE_GRID, PHASE_DENSITY, PHASE_FORCES, PHASE_COLLISIONS, PHASE_INTEGRATE = range(5)
-FLAG_DIAGNOSTICS, FLAG_PROFILE = 1, 2
+FLAG_DIAGNOSTICS, FLAG_PROFILE, FLAG_REFERENCE_ORDER = 1, 2, 4
 NEIGHBOUR_LISTS_OFF = 0xFFFFFFFF
 (DEBUG_CELL_PRESORT, DEBUG_PERM, DEBUG_CELL, DEBUG_CELL_START, DEBUG_NEIGHBOURS, DEBUG_DENSITY,
  DEBUG_PRESSURE, DEBUG_ACCEL, DEBUG_IDS) = range(9)

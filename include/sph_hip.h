@@ -15,7 +15,8 @@
  *  - Positions/velocities cross the ABI as packed xyz float triples (AoS, 12 B), the
  *    order of CParticle::Physics' position/velocity (include/CParticle.h:25-26).
  *  - "Array order" is the device's current particle order: stable-sorted by cell as of
- *    the last grid phase. `ids` give each slot's particle id.
+ *    the last grid phase (with SPH_FLAG_REFERENCE_ORDER: the reference's buckets, cell by
+ *    cell). `ids` give each slot's particle id.
  *  - Physics constants are the reference's compile-time values (include/CParticle.h:80-84,
  *    include/CCollisionGeometry.h:20-21); the arithmetic follows CCPUParticleSimulator
  *    (src/CCPUParticleSimulator.cpp) term by term in a fixed canonical summation order.
@@ -54,7 +55,13 @@ enum {
 /* Context flags. */
 enum {
     SPH_FLAG_DIAGNOSTICS = 1u << 0, /* keep neighbour counts and accelerations for sph_debug */
-    SPH_FLAG_PROFILE = 1u << 1      /* time every stage with HIP events (sph_stage_times) */
+    SPH_FLAG_PROFILE = 1u << 1,     /* time every stage with HIP events (sph_stage_times) */
+    SPH_FLAG_REFERENCE_ORDER = 1u << 2 /* sum in the reference's own order: CCPUParticleSimulator's swap-remove
+                                          buckets (src/CCPUParticleSimulator.cpp:37-88), cells visited x outer,
+                                          z inner (:108-121); uploaded and emitted particles join the back of
+                                          cell (0,0,0)'s bucket as addParticle does (src/CBaseParticleSimulator.cpp:
+                                          67-74). Cells are then numbered z + y*Z + x*Y*Z (the reference's
+                                          traversal order) in sph_debug. Slower grid phase; same arithmetic. */
 };
 
 /* sph_debug kinds. Per-particle arrays are in the current array order unless noted. */

@@ -6,11 +6,11 @@ Bars (stated per test):
   * density, acceleration, positions and velocities: bit-exact against the
     oracle in its canonical summation order (the order the HIP path is
     specified in), for single phases and over 100+ steps;
-  * against the oracle in the reference's own summation order: density within
-    1e-5 relative, acceleration within 1e-5 * S_i per component (S_i = the sum
-    of absolute force terms, emitted by the oracle), positions within 1e-3*h
-    for the first 10 steps, and at step 100 exactly where the canonical-order
-    oracle sits (the whole difference is summation-order chaos).
+  * the canonical order against the oracle in the reference's own summation
+    order: density within 1e-5 relative, acceleration within 1e-5 * S_i per
+    component (S_i = the sum of absolute force terms, emitted by the oracle),
+    positions within 1e-3*h for the first 10 steps (the reference-order mode is
+    bit-exact against it: test_gpu_reference_order.py).
 """
 import numpy as np
 import pytest
@@ -146,10 +146,12 @@ def test_fused_step_equals_phases(hip, orc):
     assert_bits_equal(va, vb, "velocity")
 
 
-def test_100_steps_bit_exact_and_chaos_floor(hip, orc):
-    """100 fused steps of the 4,096-particle dam break: bitwise equal to the
-    canonical-order oracle; against the reference-order oracle, within 1e-3*h at
-    step 10, and at step 100 exactly where the canonical oracle sits."""
+def test_100_steps_bit_exact_canonical(hip, orc):
+    """100 fused steps of the 4,096-particle dam break in the canonical order:
+    bitwise equal to the canonical-order oracle at steps 10 and 100. Against the
+    reference-order oracle the canonical order stays within 1e-3*h only for the
+    first 10 steps; the 1e-3*h bound at step 100 is met by the reference-order
+    mode (test_gpu_reference_order.py), whose distance there is 0."""
     pos = lattice_4k(hip)
     sim = gpu_sim(hip, 0.4, pos, diag=False)
     can = oracle_sim(orc, 0.4, pos)
@@ -168,11 +170,9 @@ def test_100_steps_bit_exact_and_chaos_floor(hip, orc):
     cs = can.state()
     assert_bits_equal(p100, cs["pos"], "positions at step 100")
     assert_bits_equal(v100, cs["vel"], "velocities at step 100")
-    floor = np.abs(cs["pos"].astype(np.float64) - ref.state()["pos"]).max()
     d100 = np.abs(p100.astype(np.float64) - ref.state()["pos"]).max()
-    print("step 100: |gpu - reference order| = %.3e m, oracle chaos floor = %.3e m (1e-3 h = %.3e)"
-          % (d100, floor, 1e-3 * float(H)))
-    assert d100 == floor
+    print("step 100: |gpu(canonical order) - oracle(reference order)| = %.3e m = %.2f h (summation-order chaos)"
+          % (d100, d100 / float(H)))
 
 
 def test_reference_dam_break_scene(hip, orc):
