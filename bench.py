@@ -257,7 +257,7 @@ def bench_fountain(sphhip, steps, host_steps):
     return out
 
 
-def bench_serving(sphhip, seconds, rate, budget_gb, seed=4):
+def bench_serving(sphhip, seconds, rate, budget_gb, seed=4, graphs=True):
     budget = int(budget_gb * 2 ** 30)
     submits = [0]
 
@@ -266,7 +266,7 @@ def bench_serving(sphhip, seconds, rate, budget_gb, seed=4):
         if submits[0] % 20 == 0:
             print("bench: serving %s" % msg, file=sys.stderr, flush=True)
 
-    with sphhip.Server(budget, max_jobs=1024, chunk_steps=8) as srv:
+    with sphhip.Server(budget, max_jobs=1024, chunk_steps=8, graphs=graphs) as srv:
         recs, st = sphhip.serving.poisson_run(srv, rate, seconds, seed=seed, log=log)
     lat = np.array([r["latency"] for r in recs]) if recs else np.zeros(1)
     return {"workload": "Poisson arrivals at %.2f jobs/s for %.0f s (seed %d): dam break, fountain or cuboid drop, "
@@ -275,7 +275,8 @@ def bench_serving(sphhip, seconds, rate, budget_gb, seed=4):
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "max_latency_s": float(lat.max()),
             "value": st["particle_steps"] / st["elapsed_s"], "unit": "particle-steps/s",
-            "busy_fraction": st["busy_s"] / st["elapsed_s"], "hbm_budget_bytes": budget, "pool_slots": st["pool"],
+            "busy_fraction": st["busy_s"] / st["elapsed_s"], "admin_fraction": st["admin_s"] / st["elapsed_s"],
+            "graphs": bool(graphs), "hbm_budget_bytes": budget, "pool_slots": st["pool"],
             "elapsed_s": st["elapsed_s"]}
 
 
@@ -296,6 +297,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="0 skips the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-graphs", type=int, default=1, help="1: the server replays its steps from HIP graphs")
     ap.add_argument("--only", choices=("batched", "fountain", "serving"),
                     help="run only this part and print its object (for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -321,7 +323,8 @@ def main():
         elif args.only == "fountain":
             part = bench_fountain(sphhip, args.fountain_steps, min(512, args.fountain_steps))
         else:
-            part = bench_serving(sphhip, args.serve_seconds, args.serve_rate, args.serve_budget_gb)
+            part = bench_serving(sphhip, args.serve_seconds, args.serve_rate, args.serve_budget_gb,
+                                 graphs=bool(args.serve_graphs))
         print(json.dumps({args.only: part}), flush=True)
         return
 
@@ -432,7 +435,8 @@ def main():
         line["fountain"] = bench_fountain(sphhip, args.fountain_steps, min(512, args.fountain_steps))
     if args.serve_seconds:
         print("bench: serving for %.0f s" % args.serve_seconds, file=sys.stderr, flush=True)
-        line["serving"] = bench_serving(sphhip, args.serve_seconds, args.serve_rate, args.serve_budget_gb)
+        line["serving"] = bench_serving(sphhip, args.serve_seconds, args.serve_rate, args.serve_budget_gb,
+                                        graphs=bool(args.serve_graphs))
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on %d steps" % args.cpu_steps, file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(pos, box, args.cpu_steps)

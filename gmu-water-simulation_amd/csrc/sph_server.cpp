@@ -4,12 +4,17 @@
 //
 // One scheduler thread owns the context and makes every HIP call. Each round it
 // (1) admits queued jobs in arrival order while the pool has a free range for
-// them (a job that does not fit yet blocks those behind it, so no job starves),
-// (2) advances all running jobs together by min(chunk, fewest steps left) steps
-// and (3) retires the jobs whose steps are done: downloads their state, frees
-// their slots. A job is stepped exactly `steps` times in the batch, and a scene
-// of a batch computes what it computes alone, so its result does not depend on
-// the other jobs.
+// them; a job that does not fit yet may be passed by later jobs that fit
+// (backfill) until it has been passed max_bypass times, after which admission
+// waits for it, so no job starves; (2) advances all running jobs together by
+// min(chunk, fewest steps left) steps and (3) retires the jobs whose steps are
+// done: reads their state back by id, frees their slots. A job is stepped
+// exactly `steps` times in the batch, and a scene of a batch computes what it
+// computes alone, so its result does not depend on the other jobs.
+//
+// Jobs are shared between the job table, the queue, the running list and any
+// thread waiting for them, so a job collected by one thread stays valid for
+// another that is still waiting on it.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -39,9 +44,12 @@ struct Job {
     int state = SPH_JOB_QUEUED;
     uint32_t scene = 0;
     uint32_t remaining = 0;
+    uint32_t bypassed = 0;          // later jobs admitted while this one did not fit
     double submit_s = 0, admit_s = 0, done_s = 0;
     std::string error;
 };
+
+using JobPtr = std::shared_ptr<Job>;
 
 }  // namespace
 
@@ -62,30 +70,43 @@ struct sph_server {
     bool stop = false;
     std::string broken;   // why the scheduler stopped on a host error (empty: running)
     uint64_t next_id = 1;
-    std::deque<Job *> queue;
-    std::vector<Job *> running;
-    std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
+    std::deque<JobPtr> queue;
+    std::vector<JobPtr> running;
+    std::unordered_map<uint64_t, JobPtr> jobs;
     sph_server_stats st{};
     uint32_t pool = 0;
+    uint32_t max_bypass = 32;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
     void run_loop();
+    void fail_all(const std::string &why);   // with mu held
 };
 
 namespace {
 
-// Admits queued jobs in order while they fit; returns false on a device error.
-bool admit(sph_server *s, std::unique_lock<std::mutex> &lk)
+// Admits queued jobs in arrival order while they fit. A job that does not fit
+// yet (no free range of slots or no free scene id) is passed by the later jobs
+// that do, each of which counts as one bypass of every job still waiting before
+// it; once a waiting job has been bypassed max_bypass times nothing behind it is
+// admitted until it fits.
+void admit(sph_server *s, std::unique_lock<std::mutex> &lk)
 {
-    while (!s->queue.empty()) {
-        Job *j = s->queue.front();
+    std::vector<JobPtr> waiting;   // jobs before the current one that did not fit
+    size_t i = 0;
+    while (i < s->queue.size()) {
+        for (const JobPtr &w : waiting)
+            if (w->bypassed >= s->max_bypass) return;   // it holds the line
+        JobPtr j = s->queue[i];
         lk.unlock();
         uint32_t scene = 0;
         sph_status st = sph_scene_add(s->ctx, &j->desc, &scene);
-        if (st == SPH_ERR_CAPACITY) {   // no free range or id yet: wait for retirements
+        if (st == SPH_ERR_CAPACITY) {
             lk.lock();
-            return true;
+            if (j->bypassed >= s->max_bypass) return;   // it holds the line
+            waiting.push_back(j);
+            ++i;
+            continue;
         }
         const bool added = st == SPH_OK;
         std::string err;
@@ -97,14 +118,16 @@ bool admit(sph_server *s, std::unique_lock<std::mutex> &lk)
             if (added) (void) sph_scene_remove(s->ctx, scene);
         }
         lk.lock();
-        s->queue.pop_front();
+        s->queue.erase(s->queue.begin() + (std::ptrdiff_t) i);
         if (st) {
             j->state = SPH_JOB_FAILED;
             j->error = err;
+            j->done_s = s->now();
             s->st.failed++;
             s->cv_done.notify_all();
             continue;
         }
+        for (const JobPtr &w : waiting) w->bypassed++;
         j->scene = scene;
         j->remaining = j->steps;
         j->state = SPH_JOB_RUNNING;
@@ -113,21 +136,23 @@ bool admit(sph_server *s, std::unique_lock<std::mutex> &lk)
         std::vector<float>().swap(j->vel);
         s->running.push_back(j);
     }
-    return true;
 }
 
-void retire(sph_server *s, Job *j)
+// Reads a finished job's particles back by id (the device writes them into the
+// library's pinned buffer) and frees its slots. False if the slots could not be
+// freed (the context is unusable).
+bool retire(sph_server *s, const JobPtr &j)
 {
+    const float *pp = nullptr, *vp = nullptr;
     uint32_t n = 0;
-    sph_status st = sph_scene_count(s->ctx, j->scene, &n);
+    sph_status st = sph_scene_readback(s->ctx, j->scene, 1, &pp, &vp, &n);
     std::vector<float> pos, vel;
     if (!st) {
-        pos.resize(3 * (size_t) n);
-        vel.resize(3 * (size_t) n);
-        st = sph_scene_download(s->ctx, j->scene, pos.data(), vel.data(), nullptr, n, 1);
+        pos.assign(pp, pp + 3 * (size_t) n);
+        vel.assign(vp, vp + 3 * (size_t) n);
     }
     const std::string err = st ? sph_last_error() : "";
-    (void) sph_scene_remove(s->ctx, j->scene);
+    const bool freed = sph_scene_remove(s->ctx, j->scene) == SPH_OK;
     std::lock_guard<std::mutex> g(s->mu);
     j->done_s = s->now();
     if (st) {
@@ -142,9 +167,32 @@ void retire(sph_server *s, Job *j)
         s->st.completed++;
     }
     s->cv_done.notify_all();
+    return freed;
 }
 
 }  // namespace
+
+// Fails every queued and running job and refuses new ones (mu held).
+void sph_server::fail_all(const std::string &why)
+{
+    broken = why;
+    for (const JobPtr &j : queue) {
+        j->state = SPH_JOB_FAILED;
+        j->error = broken;
+        j->done_s = now();
+        st.failed++;
+    }
+    for (const JobPtr &j : running) {
+        if (j->state != SPH_JOB_RUNNING) continue;
+        j->state = SPH_JOB_FAILED;
+        j->error = broken;
+        j->done_s = now();
+        st.failed++;
+    }
+    queue.clear();
+    running.clear();
+    cv_done.notify_all();
+}
 
 // The scheduler thread. A C++ exception in it (a host allocation failure) fails
 // every queued and running job and stops the server instead of ending the process.
@@ -155,20 +203,7 @@ void sph_server::run()
     } catch (...) {
         (void) sph::host_exception();
         std::lock_guard<std::mutex> g(mu);
-        broken = std::string("the scheduler stopped: ") + sph_last_error();
-        for (Job *j : queue) {
-            j->state = SPH_JOB_FAILED;
-            j->error = broken;
-            st.failed++;
-        }
-        for (Job *j : running) {
-            j->state = SPH_JOB_FAILED;
-            j->error = broken;
-            st.failed++;
-        }
-        queue.clear();
-        running.clear();
-        cv_done.notify_all();
+        fail_all(std::string("the scheduler stopped: ") + sph_last_error());
     }
 }
 
@@ -179,25 +214,28 @@ void sph_server::run_loop()
     while (true) {
         cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
         if (stop) break;
+        const double t_adm = now();
         admit(this, lk);
+        st.admin_s += now() - t_adm;
         if (running.empty()) {
-            // the head of the queue does not fit an empty pool: it never will
+            // the first queued job does not fit an empty pool: it never will
             if (!queue.empty()) {
-                Job *j = queue.front();
+                JobPtr j = queue.front();
                 queue.pop_front();
                 j->state = SPH_JOB_FAILED;
                 j->error = "job does not fit the pool";
+                j->done_s = now();
                 st.failed++;
                 cv_done.notify_all();
             }
             continue;
         }
         uint32_t k = chunk;
-        for (Job *j : running) k = std::min(k, j->remaining);
-        std::vector<Job *> batch = running;
+        for (const JobPtr &j : running) k = std::min(k, j->remaining);
+        std::vector<JobPtr> batch = running;
         lk.unlock();
         uint64_t live = 0;
-        for (Job *j : batch) {
+        for (const JobPtr &j : batch) {
             uint32_t n = 0;
             if (!sph_scene_count(ctx, j->scene, &n)) live += n;
         }
@@ -206,34 +244,42 @@ void sph_server::run_loop()
         if (!s1) s1 = sph_sync(ctx);
         const double t_b = now();
         const std::string err = s1 ? sph_last_error() : "";
-        std::vector<Job *> finished;
-        for (Job *j : batch) {
-            j->remaining = s1 ? 0 : j->remaining - k;
-            if (j->remaining == 0) finished.push_back(j);
-        }
+        bool usable = true;
         if (s1) {
+            // the chunk failed: its jobs fail and give their slots back
+            for (const JobPtr &j : batch) usable = sph_scene_remove(ctx, j->scene) == SPH_OK && usable;
             std::lock_guard<std::mutex> g(mu);
-            for (Job *j : batch) {
+            for (const JobPtr &j : batch) {
                 j->state = SPH_JOB_FAILED;
                 j->error = err;
+                j->done_s = now();
                 st.failed++;
             }
         } else {
-            for (Job *j : finished) retire(this, j);
+            for (const JobPtr &j : batch) {
+                j->remaining -= k;
+                if (j->remaining == 0) usable = retire(this, j) && usable;
+            }
         }
+        const double t_c = now();
         uint32_t free_slots = 0;
         (void) sph_batch_info(ctx, nullptr, nullptr, &free_slots, nullptr);
         lk.lock();
         st.free_slots = free_slots;
         st.busy_s += t_b - t_a;
+        st.admin_s += t_c - t_b;
         if (!s1) {
             st.batch_steps += k;
             st.particle_steps += live * k;
         }
         running.erase(std::remove_if(running.begin(), running.end(),
-                                     [](Job *j) { return j->state != SPH_JOB_RUNNING; }),
+                                     [](const JobPtr &j) { return j->state != SPH_JOB_RUNNING; }),
                       running.end());
         cv_done.notify_all();
+        if (!usable) {
+            fail_all("the scheduler stopped: a failed job's slots could not be freed: " + err);
+            break;
+        }
     }
 }
 
@@ -263,6 +309,7 @@ sph_status sph_server_create(const sph_server_params *p, sph_server **out)
         }
         if (p->graphs) (void) sph_set_graphs(s->ctx, 1);
         s->pool = (uint32_t) slots;
+        s->max_bypass = p->max_bypass ? p->max_bypass : 32u;
         s->st.pool = s->pool;
         s->st.free_slots = s->pool;
         sph_server *raw = s.release();
@@ -301,7 +348,7 @@ sph_status sph_server_submit(sph_server *s, const sph_job_desc *d, uint64_t *job
         if ((uint64_t) d->scene.capacity > s->pool)
             return report(SPH_ERR_CAPACITY, "sph_server_submit: capacity " + std::to_string(d->scene.capacity) +
                                                 " above the pool of " + std::to_string(s->pool) + " slots");
-        auto j = std::make_unique<Job>();
+        auto j = std::make_shared<Job>();
         j->desc = d->scene;
         j->n = d->n;
         j->steps = d->steps;
@@ -316,8 +363,8 @@ sph_status sph_server_submit(sph_server *s, const sph_job_desc *d, uint64_t *job
         j->id = s->next_id++;
         j->submit_s = s->now();
         *job_id = j->id;
-        s->queue.push_back(j.get());
-        s->jobs[j->id] = std::move(j);
+        s->queue.push_back(j);
+        s->jobs[j->id] = j;
         s->st.submitted++;
         s->cv_work.notify_all();
         return SPH_OK;
@@ -356,7 +403,7 @@ sph_status sph_server_wait(sph_server *s, uint64_t id, double timeout_s, sph_job
         std::unique_lock<std::mutex> lk(s->mu);
         auto it = s->jobs.find(id);
         if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_wait: unknown job " + std::to_string(id));
-        Job *j = it->second.get();
+        const JobPtr j = it->second;   // stays valid if another thread collects the job meanwhile
         auto done = [&] { return j->state == SPH_JOB_DONE || j->state == SPH_JOB_FAILED; };
         if (timeout_s < 0)
             s->cv_done.wait(lk, done);
@@ -378,6 +425,11 @@ sph_status sph_server_result(sph_server *s, uint64_t id, float *pos3, float *vel
         auto it = s->jobs.find(id);
         if (it == s->jobs.end()) return report(SPH_ERR_INVALID, "sph_server_result: unknown job " + std::to_string(id));
         Job &j = *it->second;
+        if (j.state == SPH_JOB_FAILED) {
+            const std::string why = "job " + std::to_string(id) + " failed: " + j.error;
+            s->jobs.erase(it);
+            return report(SPH_ERR_FAULT, why);
+        }
         if (j.state != SPH_JOB_DONE) return report(SPH_ERR_STATE, "sph_server_result: job not done");
         if (n != j.n) return report(SPH_ERR_INVALID, "sph_server_result: n = " + std::to_string(n) + ", job holds " +
                                                         std::to_string(j.n));

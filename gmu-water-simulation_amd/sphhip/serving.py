@@ -18,7 +18,7 @@ JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(4)
 class sph_server_params(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_uint32), ("chunk_steps", C.c_uint32),
                 ("max_neighbours", C.c_uint32), ("flags", C.c_uint32), ("variant", C.c_int32),
-                ("graphs", C.c_uint32)]
+                ("graphs", C.c_uint32), ("max_bypass", C.c_uint32)]
 
 
 class sph_job_desc(C.Structure):
@@ -35,7 +35,7 @@ class sph_server_stats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("queued", C.c_uint64), ("running", C.c_uint64), ("particle_steps", C.c_uint64),
                 ("batch_steps", C.c_uint64), ("busy_s", C.c_double), ("elapsed_s", C.c_double),
-                ("pool", C.c_uint32), ("free_slots", C.c_uint32)]
+                ("pool", C.c_uint32), ("free_slots", C.c_uint32), ("admin_s", C.c_double)]
 
 
 def _load():
@@ -50,10 +50,10 @@ class Server:
     """A serving loop over one batch context of `hbm_budget` device bytes."""
 
     def __init__(self, hbm_budget, max_jobs=1024, chunk_steps=8, max_neighbours=0, variant=-1, graphs=False,
-                 flags=0):
+                 flags=0, max_bypass=0):
         self._lib = _load()
         p = sph_server_params(int(hbm_budget), int(max_jobs), int(chunk_steps), int(max_neighbours), int(flags),
-                              int(variant), int(bool(graphs)))
+                              int(variant), int(bool(graphs)), int(max_bypass))
         srv = C.c_void_p()
         _lib.check(self._lib.sph_server_create(C.byref(p), C.byref(srv)))
         self._srv = srv

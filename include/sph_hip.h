@@ -407,6 +407,8 @@ typedef struct {
     uint32_t flags;            /* SPH_FLAG_* of the context */
     int32_t variant;           /* neighbour-kernel variant, -1 = default */
     uint32_t graphs;           /* != 0: replay steps from HIP graphs */
+    uint32_t max_bypass;       /* a queued job that does not fit yet may be passed by this many later jobs
+                                  that do before admission waits for it (0 = 32) */
 } sph_server_params;
 
 typedef struct {
@@ -433,6 +435,7 @@ typedef struct {
     double busy_s;             /* wall time inside step chunks */
     double elapsed_s;          /* server clock now */
     uint32_t pool, free_slots;
+    double admin_s;            /* wall time admitting (uploads, layout) and retiring (readback) jobs */
 } sph_server_stats;
 
 sph_status sph_server_create(const sph_server_params *params, sph_server **out);
@@ -443,7 +446,8 @@ sph_status sph_server_submit(sph_server *srv, const sph_job_desc *job, uint64_t 
 sph_status sph_server_job(sph_server *srv, uint64_t job_id, sph_job_info *info);
 /* Waits until the job is done or failed (timeout_s < 0: no limit); *info as sph_server_job. */
 sph_status sph_server_wait(sph_server *srv, uint64_t job_id, double timeout_s, sph_job_info *info);
-/* Copies a done job's particles by id (n = its final count) and forgets the job. */
+/* Copies a done job's particles by id (n = its final count) and forgets the job. A failed
+ * job is forgotten too and returns SPH_ERR_FAULT with its error. */
 sph_status sph_server_result(sph_server *srv, uint64_t job_id, float *pos3, float *vel3, uint32_t n);
 sph_status sph_server_stats_get(sph_server *srv, sph_server_stats *stats);
 

@@ -413,6 +413,24 @@ def test_readback_by_id_equals_download(hip):
     sim.close()
 
 
+def test_readback_by_id_equals_oracle(hip, orc):
+    """sph_scene_readback of each scene of a batch against the canonical-order
+    oracle's state by id (the render refresh of CGPUBaseParticleSimulator::integrate,
+    src/CGPUBaseParticleSimulator.cpp:83-91)."""
+    scs = small_scenes(hip, count=3, seed=41)
+    with make_batch(hip, scs) as b:
+        b.step(7)
+        for i, sc in enumerate(scs):
+            rp, rv = b.readback(i)
+            o = orc.OracleSim(sc["box"], sc["n"])
+            o.add_particles(sc["pos"])
+            o.set_order_mode(orc.ORDER_CANONICAL)
+            o.step(7)
+            st = o.state()
+            assert_bits_equal(rp, st["pos"], "readback positions scene %d" % i)
+            assert_bits_equal(rv, st["vel"], "readback velocities scene %d" % i)
+
+
 def test_readback_rejects_bad_ids(hip):
     pos = hip.scenes.lattice(4, 4, 4, 0.3, seed=2)
     sim = hip.HIPGridParticleSimulator(0.3, capacity=len(pos))
