@@ -725,7 +725,7 @@ sph_status run_density(sph_ctx *c)
     if (t.st) return t.st;
     const bool diag = (c->flags & SPH_FLAG_DIAGNOSTICS) != 0;
     sph::launch_density(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->fields(diag), c->nlists(), c->table(),
-                        c->k, c->variant, c->stream);
+                        c->k, c->variant, c->ref_order(), c->stream);
     return launch_error("density launch");
 }
 
@@ -738,7 +738,7 @@ sph_status run_fused_tail(sph_ctx *c)
     sph::ParticleSoA out = c->soa(1 - c->cur, c->id_cur);
     sph::launch_forces_fused(in, c->key_s, c->cstart, c->fields(diag), c->nlists(), out, c->key_h,
                              diag ? c->acc[0] : nullptr, diag ? c->acc[1] : nullptr, diag ? c->acc[2] : nullptr,
-                             c->table(), c->k, c->variant, c->stream);
+                             c->table(), c->k, c->variant, c->ref_order(), c->stream);
     SPH_TRY(launch_error("fused forces launch"));
     c->cur = 1 - c->cur;
     c->keys_valid = true;
@@ -828,7 +828,7 @@ sph_status run_phase(sph_ctx *c, int phase)
         StageTimer t(c, SPH_STAGE_FORCES);
         if (t.st) return t.st;
         sph::launch_forces(c->soa(c->cur, c->id_cur), c->key_s, c->cstart, c->fields(false), c->nlists(), c->acc[0],
-                           c->acc[1], c->acc[2], c->table(), c->k, c->stream);
+                           c->acc[1], c->acc[2], c->table(), c->k, c->ref_order(), c->stream);
         st = launch_error("forces launch");
         break;
     }

@@ -162,14 +162,16 @@ void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32
 void launch_scatter_by_id(const ParticleSoA &p, uint32_t first, uint32_t n, float *pos, float *vel, uint32_t *bad,
                           hipStream_t st);
 
-// Density pass: density, p/rho^2, 1/rho and the neighbour lists.
+// Density pass: density, p/rho^2, 1/rho and the neighbour lists. `ref`: the
+// scenes use the reference order (kOrderReference; one kernel shape).
 void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
-                    const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, hipStream_t st);
+                    const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, bool ref,
+                    hipStream_t st);
 
 // Pressure + viscosity + gravity acceleration into acc (the updateForces phase).
 void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
                    const NeighbourList &nl, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
-                   hipStream_t st);
+                   bool ref, hipStream_t st);
 
 // acc += wall response (the updateCollisions phase).
 void launch_collide(const ParticleSoA &p, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
@@ -184,6 +186,6 @@ void launch_integrate(const ParticleSoA &p, const float *ax, const float *ay, co
 void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
                          const DensityFields &f, const NeighbourList &nl, const ParticleSoA &out,
                          uint32_t *key_next, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
-                         int variant, hipStream_t st);
+                         int variant, bool ref, hipStream_t st);
 
 }  // namespace sph

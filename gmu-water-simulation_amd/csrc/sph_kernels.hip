@@ -40,19 +40,27 @@ __device__ __forceinline__ int clamp_cell(int c, int r)
 // Local cell number of grid cell (x, y, z): x + y*rx + z*rx*ry (include/CGrid.h:25-28)
 // in the canonical order, z + y*rz + x*rz*ry (the reference's traversal order,
 // src/CCPUParticleSimulator.cpp:37-39) in the reference order.
+template <bool kRef>
 __device__ __forceinline__ uint32_t cell_number(int x, int y, int z, const Scene &s)
 {
-    return (uint32_t) (s.order == kOrderReference ? z + y * s.rz + x * s.rz * s.ry : x + y * s.rx + z * s.rx * s.ry);
+    return (uint32_t) (kRef ? z + y * s.rz + x * s.rz * s.ry : x + y * s.rx + z * s.rx * s.ry);
 }
 
 // Cell hash, src/CCPUParticleSimulator.cpp:46-69:
 // floor(((double)p + (double)box/2.0) / (double)h), clamped, numbered as cell_number.
+template <bool kRef>
 __device__ __forceinline__ uint32_t hash_key(float px, float py, float pz, const Scene &s, double h)
 {
     const int x = clamp_cell(to_int_x86(floor(((double) px + (double) s.bx / 2.0) / h)), s.rx);
     const int y = clamp_cell(to_int_x86(floor(((double) py + (double) s.by / 2.0) / h)), s.ry);
     const int z = clamp_cell(to_int_x86(floor(((double) pz + (double) s.bz / 2.0) / h)), s.rz);
-    return s.coff + cell_number(x, y, z, s);
+    return s.coff + cell_number<kRef>(x, y, z, s);
+}
+
+// hash_key in the scene's own order (kernels that are not specialised on it).
+__device__ __forceinline__ uint32_t hash_key_any(float px, float py, float pz, const Scene &s, double h)
+{
+    return s.order == kOrderReference ? hash_key<true>(px, py, pz, s, h) : hash_key<false>(px, py, pz, s, h);
 }
 
 // CCollisionGeometry::inverseBoundingBoxBounce (src/CCollisionGeometry.cpp:117-132)
@@ -115,10 +123,11 @@ __device__ __forceinline__ uint32_t dead_key(const Scene &s)
     return s.coff + s.ncells;
 }
 
+template <bool kRef>
 __device__ __forceinline__ void cell_xyz(uint32_t key, const Scene &s, int &x, int &y, int &z)
 {
     const uint32_t c = key - s.coff;
-    if (s.order == kOrderReference) {
+    if constexpr (kRef) {
         z = (int) (c % (uint32_t) s.rz);
         const uint32_t rem = c / (uint32_t) s.rz;
         y = (int) (rem % (uint32_t) s.ry);
@@ -140,7 +149,7 @@ __global__ __launch_bounds__(256) void k_hash(const float *__restrict__ px, cons
     const uint32_t first = blockIdx.x * 256u, i = first + threadIdx.x;
     const Scene s = block_scene(t, first);
     if (i >= t.nslots) return;
-    key[i] = (i - s.poff < s.n) ? hash_key(px[i], py[i], pz[i], s, h) : dead_key(s);
+    key[i] = (i - s.poff < s.n) ? hash_key_any(px[i], py[i], pz[i], s, h) : dead_key(s);
 }
 
 // SplitMix64 output `k` (k >= 1) of the stream that starts at `state`.
@@ -218,7 +227,7 @@ __global__ __launch_bounds__(256) void k_emit(const Emitter *__restrict__ em, Sc
         p.vy[slot] = vy;
         p.vz[slot] = 0.0f;
         p.id[slot] = g;
-        key[slot] = hash_key(x, floor_y, z, s, h);
+        key[slot] = hash_key_any(x, floor_y, z, s, h);
         if (ocell) ocell[slot] = 0u;   // addParticle puts it at the end of cell (0,0,0)'s bucket
     }
     __syncthreads();
@@ -792,6 +801,7 @@ __device__ __forceinline__ RowBounds row_bounds(const Scene &s, const Consts &k,
 
 // [j, jend): the candidate slots of row r, relative to the scene's first slot
 // (empty when the row is off the grid or culled).
+template <bool kRef>
 __device__ __forceinline__ void row_run(const RowBounds &rb, const Scene &s, const uint32_t *__restrict__ start, int x,
                                         int y, int z, int r, uint32_t &j, uint32_t &jend)
 {
@@ -799,7 +809,7 @@ __device__ __forceinline__ void row_run(const RowBounds &rb, const Scene &s, con
     jend = 0;
     const int oy = r % 3 - 1, yy = y + oy;
     const double ly = oy < 0 ? rb.by.lo2 : oy > 0 ? rb.by.hi2 : 0.0;
-    if (s.order == kOrderReference) {
+    if constexpr (kRef) {
         const int ox = r / 3 - 1, xx = x + ox;
         const double lxy = ly + (ox < 0 ? rb.bx.lo2 : ox > 0 ? rb.bx.hi2 : 0.0);
         if (xx >= 0 && xx < s.rx && yy >= 0 && yy < s.ry && lxy <= rb.lim) {
@@ -837,7 +847,7 @@ struct NoRowHooks {
 // full, flush() drains every lane's list. hooks.trip(j0, bits) gets the accept
 // bits of each trip (bit u: candidate j0 + u). kWholeTrips: a trip in which
 // every lane has U candidates left skips the per-candidate run-end test.
-template <int U, bool kWholeTrips = false, int kCap = kListCap, typename Test, typename Flush,
+template <bool kRef, int U, bool kWholeTrips = false, int kCap = kListCap, typename Test, typename Flush,
           typename Hooks = NoRowHooks>
 __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k, const uint32_t *__restrict__ start,
                                                 const BufView &bv, int x, int y, int z, float xi, float yi,
@@ -847,7 +857,7 @@ __device__ __forceinline__ void walk_neighbours(const Scene &s, const Consts &k,
     const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
     for (int r = 0; r < 9; ++r) {
         uint32_t j, jend;
-        row_run(rb, s, start, x, y, z, r, j, jend);
+        row_run<kRef>(rb, s, start, x, y, z, r, j, jend);
         hooks.row_begin(r, j, jend);
         while (__any(j < jend)) {
             if (j < jend) {
@@ -931,26 +941,16 @@ struct MaskRecorder {
     }
 };
 
-// One term of updateDensityPressure (src/CCPUParticleSimulator.cpp:126):
-// density += Wpoly6(r^2) in double, rounded to the float density, with
-// Wpoly6 = poly6 * pow(h^2 - r^2, 3) as the C library's pow gives it
-// (src/CCPUParticleSimulator.cpp:14): the correctly rounded cube, and the pow
-// table only where the float result can depend on it (near_float_boundary).
-__device__ __forceinline__ double density_sum_lib(float density, float r2, const Consts &k)
-{
-    const double t = k.hsq - (double) r2;
-    double c = cube_cr(t);
-    const int d = pow_exception(g_pow3_exc, g_pow3_exc_count, __float_as_uint(r2));
-    if (d) c = next_double(c, d);
-    return (double) density + k.poly6 * c;
-}
-
+// One term of updateDensityPressure and the spiky gradient's factor, with the
+// C library's pow (see density_term, spiky_factor).
 __device__ __forceinline__ float density_add(float density, float r2, const Consts &k)
 {
-    const double wpoly = k.poly6 * cube_cr(k.hsq - (double) r2);
-    double sum = (double) density + wpoly;
-    if (__builtin_expect(near_float_boundary(sum), 0)) sum = density_sum_lib(density, r2, k);
-    return (float) sum;
+    return density_term(density, r2, k.hsq, k.poly6, g_pow3_exc, g_pow3_exc_count);
+}
+
+__device__ __forceinline__ float spiky_scale(double hr, float r2, const Consts &k)
+{
+    return spiky_factor(hr, r2, k.spiky, g_pow2_exc, g_pow2_exc_count);
 }
 
 // updateDensityPressure, src/CCPUParticleSimulator.cpp:93-141. Also stores
@@ -959,7 +959,7 @@ __device__ __forceinline__ float density_add(float density, float r2, const Cons
 // neighbours (minus itself), with kMasks its rows' accepted-candidate masks
 // (masks, and nl.cnt[i] = 1 when a run is too long for them); with kDiag, the
 // accepted count. T threads per workgroup, U candidates per lane per trip.
-template <int T, int U, bool kLists, bool kMasks, bool kDiag>
+template <int T, int U, bool kLists, bool kMasks, bool kDiag, bool kRef>
 __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                const uint32_t *__restrict__ start, DensityFields f,
                                                NeighbourList nl, uint32_t *__restrict__ masks, SceneTable t,
@@ -978,7 +978,7 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
     const BufView bv = buf_view(p, f, s);
     const float xi = p.px[i], yi = p.py[i], zi = p.pz[i];
     int x, y, z;
-    cell_xyz(cell[i], s, x, y, z);
+    cell_xyz<kRef>(cell[i], s, x, y, z);
     float density = 0.0f;
     uint32_t nc = 0, nn = 0;
     int cnt = 0;
@@ -1002,9 +1002,9 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
         for (int q = 0; q < cnt; ++q) density = density_add(density, my[q * T], k);
     };
     if constexpr (kMasks)
-        walk_neighbours<U, false, kCap>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush, rec_masks);
+        walk_neighbours<kRef, U, false, kCap>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush, rec_masks);
     else
-        walk_neighbours<U, false, kCap>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush);
+        walk_neighbours<kRef, U, false, kCap>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush);
     density = density * k.mass;
     const float pressure = k.gas_k * (density - k.rho0);
     f.rho[i] = density;
@@ -1031,7 +1031,7 @@ struct ForceSums {
 // The neighbour data one force pair reads, loaded one pair ahead of its use so
 // that the loads of pair q+1 are in flight while pair q is computed.
 struct PairData {
-    uint32_t j;
+    uint32_t off;   // byte offset of the neighbour's record: 40 * its slot
     float x, y, z, vx, vy, vz;
     double prr, rinv;
 };
@@ -1043,13 +1043,13 @@ __device__ __forceinline__ double own_prr(const DensityFields &f, uint32_t i)
     return __builtin_bit_cast(double, (uint64_t) w.x | ((uint64_t) w.y << 32));
 }
 
-__device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
+// The record at byte offset o (40 * slot).
+__device__ __forceinline__ PairData load_pair_at(const BufView &bv, uint32_t o)
 {
-    const uint32_t o = j * (4u * kRecWords);
     const float4 a = ld4(bv.rec, o), b = ld4(bv.rec, o + 16u);
     const uint2 c = ld2u(bv.rec, o + 32u);
     PairData d;
-    d.j = j;
+    d.off = o;
     d.x = a.x;
     d.y = a.y;
     d.z = a.z;
@@ -1061,6 +1061,11 @@ __device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
     return d;
 }
 
+__device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
+{
+    return load_pair_at(bv, j * (4u * kRecWords));
+}
+
 // The six float terms one accepted pair adds to the force sums
 // (src/CCPUParticleSimulator.cpp:174-183, with WspikyGradient /
 // WviscosityLaplacian, src/CCPUParticleSimulator.cpp:17-30): the pressure term
@@ -1069,24 +1074,6 @@ __device__ __forceinline__ PairData load_pair(const BufView &bv, uint32_t j)
 struct PairTerms {
     float px, py, pz, vx, vy, vz;
 };
-
-// (float)(spiky * pow(h - r, 2)) with the C library's pow
-// (src/CCPUParticleSimulator.cpp:21): the correctly rounded square, and the pow
-// table only where the float can depend on it.
-__device__ __forceinline__ double spiky_lib(double hr, float r2, const Consts &k)
-{
-    double sq = hr * hr;
-    const int d = pow_exception(g_pow2_exc, g_pow2_exc_count, __float_as_uint(r2));
-    if (d) sq = next_double(sq, d);
-    return k.spiky * sq;
-}
-
-__device__ __forceinline__ float spiky_scale(double hr, float r2, const Consts &k)
-{
-    double sg = k.spiky * (hr * hr);
-    if (__builtin_expect(near_float_boundary(sg), 0)) sg = spiky_lib(hr, r2, k);
-    return (float) sg;
-}
 
 __device__ __forceinline__ PairTerms pair_terms(const BufView &bv, const Consts &k, const PairData &n, float xi,
                                                 float yi, float zi, float vxi, float vyi, float vzi, double ai)
@@ -1105,7 +1092,7 @@ __device__ __forceinline__ PairTerms pair_terms(const BufView &bv, const Consts 
     t.pz = gz * pf;
     const float lap = (float) (k.visc * hr);
     div3_rcp((n.vx - vxi) * lap, (n.vy - vyi) * lap, (n.vz - vzi) * lap, n.rinv,
-             [&]() { return ldf(bv.rho, n.j * 4u); }, t.vx, t.vy, t.vz);
+             [&]() { return ldf(bv.rho, n.off / 10u); }, t.vx, t.vy, t.vz);
     return t;
 }
 
@@ -1150,7 +1137,7 @@ __device__ __forceinline__ void drain_pairs(const BufView &bv, const Consts &k, 
 // The force sums of particle i (slot li of its scene) from walking its cells,
 // in canonical order: each lane appends its accepted neighbours to its LDS list
 // `my` (stride T) and drains it through the pair body.
-template <int T, int U>
+template <bool kRef, int T, int U>
 __device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t *__restrict__ cell,
                                                const uint32_t *__restrict__ start, uint32_t *my, uint32_t i,
                                                uint32_t li, const Scene &s, const Consts &k, float xi, float yi,
@@ -1158,9 +1145,9 @@ __device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t
 {
     ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int x, y, z;
-    cell_xyz(cell[i], s, x, y, z);
+    cell_xyz<kRef>(cell[i], s, x, y, z);
     int cnt = 0;
-    walk_neighbours<U, true>(
+    walk_neighbours<kRef, U, true>(
         s, k, start, bv, x, y, z, xi, yi, zi, cnt,
         [&](uint32_t j, bool valid, float xj, float yj, float zj) {
             const float dx = xi - xj, dyv = yi - yj, dzv = zi - zj;
@@ -1182,7 +1169,7 @@ __device__ __forceinline__ ForceSums walk_sums(const BufView &bv, const uint32_t
 // lets full lanes wait (keeping their bits) until P percent of the wave waits,
 // or until every lane with bits left waits, so that drains run with most lanes
 // busy. Each lane's pairs keep their order either way.
-template <int T, int C, int kStall>
+template <bool kRef, int T, int C, int kStall>
 __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t *__restrict__ cell,
                                                const uint32_t *__restrict__ start,
                                                const uint32_t *__restrict__ masks, uint32_t *my, uint32_t i,
@@ -1191,7 +1178,7 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
 {
     ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int x, y, z;
-    cell_xyz(cell[i], s, x, y, z);
+    cell_xyz<kRef>(cell[i], s, x, y, z);
     const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
     // rows and words advance together for the wave (r, w uniform); m holds the
     // lane's unread bits of word w of row r, whose first candidate is jw
@@ -1211,7 +1198,7 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
                         break;
                     }
                     uint32_t jend;
-                    row_run(rb, s, start, x, y, z, r, jrun, jend);
+                    row_run<kRef>(rb, s, start, x, y, z, r, jrun, jend);
                     nw = (jend - jrun + kMaskWordBits - 1) / kMaskWordBits;
                     w = 0;
                 }
@@ -1249,36 +1236,38 @@ __device__ __forceinline__ ForceSums mask_sums(const BufView &bv, const uint32_t
 }
 
 // Drains a lane's queue of mask words (nq entries of two 32-bit words at
-// stride T: the bits and the slot of bit 0) into its force sums: the set bits of
-// each word in ascending order, word after word. Two pairs are in flight, as in
-// drain_pairs; a lane with no pair left loads `fallback` and adds nothing.
+// stride T: the bits and the record offset of bit 0, 40 * its slot) into its
+// force sums: the set bits of each word in ascending order, word after word. Two
+// pairs are in flight, as in drain_pairs; a lane with no pair left loads the
+// record at `fallback` (an offset) and adds nothing. A bit's record offset is
+// the word's plus a 24-bit product, so no full 32-bit multiply per pair.
 template <int T>
 __device__ __forceinline__ void drain_words(const BufView &bv, const Consts &k, const uint32_t *qw, int nq,
                                             uint32_t fallback, float xi, float yi, float zi, float vxi, float vyi,
                                             float vzi, double ai, ForceSums &a)
 {
     int q = 0;
-    uint32_t m = 0, j0 = 0;
-    auto next = [&](uint32_t &j) {
+    uint32_t m = 0, o0 = 0;
+    auto next = [&](uint32_t &o) {
         if (m == 0u && q < nq) {
             m = qw[(2 * q) * T];
-            j0 = qw[(2 * q + 1) * T];
+            o0 = qw[(2 * q + 1) * T];
             ++q;
         }
         const bool ok = m != 0u;
-        j = ok ? j0 + (uint32_t) __builtin_ctz(m) : fallback;
+        o = ok ? o0 + __umul24((uint32_t) __builtin_ctz(m), 4u * kRecWords) : fallback;
         m &= m - 1u;
         return ok;
     };
-    uint32_t ja, jb;
-    bool va = next(ja);
-    PairData pa = load_pair(bv, ja);
+    uint32_t oa, ob;
+    bool va = next(oa);
+    PairData pa = load_pair_at(bv, oa);
     while (__any(va)) {
-        const bool vb = next(jb);
-        const PairData pb = load_pair(bv, jb);
+        const bool vb = next(ob);
+        const PairData pb = load_pair_at(bv, ob);
         if (va) force_pair(bv, k, pa, xi, yi, zi, vxi, vyi, vzi, ai, a);
-        va = next(ja);
-        pa = load_pair(bv, ja);
+        va = next(oa);
+        pa = load_pair_at(bv, oa);
         if (vb) force_pair(bv, k, pb, xi, yi, zi, vxi, vyi, vzi, ai, a);
     }
 }
@@ -1292,7 +1281,7 @@ constexpr unsigned kWaitVmcnt0 = 0x0F70;
 // W entries at stride T); drain_words runs through every lane's queue when one
 // is full and at the end. With W large enough for a particle's words there is
 // one drain per wave and no per-bit fill rounds, at the price of more LDS.
-template <int T, int W>
+template <bool kRef, int T, int W>
 __device__ __forceinline__ ForceSums mask_sums_queue(const BufView &bv, const uint32_t *__restrict__ cell,
                                                      const uint32_t *__restrict__ start,
                                                      const uint32_t *__restrict__ masks, uint32_t *qw, uint32_t i,
@@ -1301,9 +1290,10 @@ __device__ __forceinline__ ForceSums mask_sums_queue(const BufView &bv, const ui
 {
     ForceSums a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int x, y, z;
-    cell_xyz(cell[i], s, x, y, z);
+    cell_xyz<kRef>(cell[i], s, x, y, z);
     const RowBounds rb = row_bounds(s, k, x, y, z, xi, yi, zi);
-    const uint32_t fallback = li + 1 < s.n ? li + 1 : li - 1;   // loaded by idle lanes, never added
+    // loaded by idle lanes, never added (a record offset)
+    const uint32_t fallback = (li + 1 < s.n ? li + 1 : li - 1) * (4u * kRecWords);
     // the first word of every row, loaded together up front (unused where a row is empty)
     uint32_t m0[9];
 #pragma unroll
@@ -1311,7 +1301,7 @@ __device__ __forceinline__ ForceSums mask_sums_queue(const BufView &bv, const ui
     int nq = 0;
     for (int r = 0; r < 9; ++r) {
         uint32_t jrun, jend;
-        row_run(rb, s, start, x, y, z, r, jrun, jend);
+        row_run<kRef>(rb, s, start, x, y, z, r, jrun, jend);
         const uint32_t nw = (jend - jrun + kMaskWordBits - 1) / kMaskWordBits;
         for (uint32_t w = 0; __any(w < nw); ++w) {
             uint32_t m = w >= nw ? 0u : w == 0 ? m0[r] : masks[mask_word(i, r, w)];
@@ -1319,7 +1309,7 @@ __device__ __forceinline__ ForceSums mask_sums_queue(const BufView &bv, const ui
             if (self < kMaskWordBits) m &= ~(1u << self);
             if (m != 0u) {
                 qw[(2 * nq) * T] = m;
-                qw[(2 * nq + 1) * T] = j0;
+                qw[(2 * nq + 1) * T] = j0 * (4u * kRecWords);
                 ++nq;
             }
             if (__any(nq == W)) {
@@ -1349,7 +1339,7 @@ __device__ __forceinline__ void accel_from_sums(const ForceSums &a, float rhoi, 
 // The acceleration of particle i. Neighbours come from the density pass's list
 // when one was kept (each lane running through its own list); otherwise (or on
 // overflow) the cells are walked again in the same order.
-template <int T, int U>
+template <bool kRef, int T, int U>
 __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *__restrict__ cell,
                                           const uint32_t *__restrict__ start, const DensityFields &f,
                                           const NeighbourList &nl, uint32_t *my, uint32_t i, uint32_t li,
@@ -1374,9 +1364,9 @@ __device__ __forceinline__ void sph_accel(const ParticleSoA &p, const uint32_t *
             force_pair(bv, k, cur, xi, yi, zi, vxi, vyi, vzi, ai, a);
         }
     } else if (nl.mode == kNbMasks && nl.cnt[i] == 0) {
-        a = mask_sums<T, kListCap, 0>(bv, cell, start, nl.masks, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
+        a = mask_sums<kRef, T, kListCap, 0>(bv, cell, start, nl.masks, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
     } else {
-        a = walk_sums<T, U>(bv, cell, start, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
+        a = walk_sums<kRef, T, U>(bv, cell, start, my, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
     }
     accel_from_sums(a, rhoi, s, k, ax, ay, az);
 }
@@ -1478,6 +1468,7 @@ __device__ __forceinline__ void integrate_one(float xi, float yi, float zi, floa
 constexpr int kPhaseThreads = 256;   // workgroup of the phase-by-phase kernels
 constexpr int kPhaseUnroll = 4;
 
+template <bool kRef>
 __global__ __launch_bounds__(kPhaseThreads) void k_forces(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                           const uint32_t *__restrict__ start, DensityFields f,
                                                           NeighbourList nl, float *__restrict__ ax,
@@ -1490,7 +1481,7 @@ __global__ __launch_bounds__(kPhaseThreads) void k_forces(ParticleSoA p, const u
     const uint32_t li = i - s.poff;
     if (i >= t.nslots || li >= s.n) return;
     float a0, a1, a2;
-    sph_accel<kPhaseThreads, kPhaseUnroll>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, p.px[i], p.py[i],
+    sph_accel<kRef, kPhaseThreads, kPhaseUnroll>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, p.px[i], p.py[i],
                                            p.pz[i], p.vx[i], p.vy[i], p.vz[i], a0, a1, a2);
     ax[i] = a0;
     ay[i] = a1;
@@ -1527,7 +1518,7 @@ __global__ __launch_bounds__(256) void k_integrate(ParticleSoA p, const float *_
                   nvx, nvy, nvz);
     out.px[i] = nx; out.py[i] = ny; out.pz[i] = nz;
     out.vx[i] = nvx; out.vy[i] = nvy; out.vz[i] = nvz;
-    key_next[i] = hash_key(nx, ny, nz, s, k.h);
+    key_next[i] = hash_key_any(nx, ny, nz, s, k.h);
 }
 
 // Forces, walls and integration of one particle in one pass, plus the cell key of
@@ -1535,7 +1526,7 @@ __global__ __launch_bounds__(256) void k_integrate(ParticleSoA p, const float *_
 // Neighbours: kNbWalk walks the cells; kNbLists takes the density pass's lists,
 // a wave at a time (wave_list_sums); kNbMasks iterates the density pass's row
 // masks (mask_sums). Particles whose list or masks overflowed walk their cells.
-template <int T, int U, int C, int kStall, bool kQueue, int kMode, bool kStoreAcc>
+template <int T, int U, int C, int kStall, bool kQueue, int kMode, bool kStoreAcc, bool kRef>
 __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_t *__restrict__ cell,
                                                     const uint32_t *__restrict__ start, DensityFields f,
                                                     NeighbourList nl, ParticleSoA out,
@@ -1561,7 +1552,7 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
     const float vxi = p.vx[i], vyi = p.vy[i], vzi = p.vz[i];
     float a0, a1, a2;
     if (kMode == kNbWalk) {
-        sph_accel<T, U>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1,
+        sph_accel<kRef, T, U>(p, cell, start, f, nl, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, a0, a1,
                         a2);
     } else {
         const BufView bv = buf_view(p, f, s);
@@ -1578,14 +1569,14 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
         } else {
             overflow = cnt_i != 0;
             if (!overflow && kQueue)
-                a = mask_sums_queue<T, C / 2>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
+                a = mask_sums_queue<kRef, T, C / 2>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
                                               vxi, vyi, vzi, ai);
             else if (!overflow)
-                a = mask_sums<T, C, kStall>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
+                a = mask_sums<kRef, T, C, kStall>(bv, cell, start, nl.masks, lst + threadIdx.x, i, li, s, k, xi, yi, zi,
                                             vxi, vyi, vzi, ai);
         }
         if (overflow)
-            a = walk_sums<T, U>(bv, cell, start, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
+            a = walk_sums<kRef, T, U>(bv, cell, start, lst + threadIdx.x, i, li, s, k, xi, yi, zi, vxi, vyi, vzi, ai);
         accel_from_sums(a, rhoi, s, k, a0, a1, a2);
     }
     wall_accel(xi, yi, zi, vxi, vyi, vzi, s, k, a0, a1, a2);
@@ -1598,13 +1589,13 @@ __global__ __launch_bounds__(T) void k_forces_fused(ParticleSoA p, const uint32_
     integrate_one(xi, yi, zi, vxi, vyi, vzi, a0, a1, a2, s.dt, nx, ny, nz, nvx, nvy, nvz);
     out.px[i] = nx; out.py[i] = ny; out.pz[i] = nz;
     out.vx[i] = nvx; out.vy[i] = nvy; out.vz[i] = nvz;
-    key_next[i] = hash_key(nx, ny, nz, s, k.h);
+    key_next[i] = hash_key<kRef>(nx, ny, nz, s, k.h);
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 
 // The compiled shapes of the two neighbour kernels, indexed by variant.
-template <int T, int U, int C, int kStall, bool kQueue>
+template <int T, int U, int C, int kStall, bool kQueue, bool kRef = false>
 struct NeighbourKernels {
     // the density pass of the word-queue variants keeps 64-thread workgroups
     // (its fastest shape); `T` then shapes the force pass only
@@ -1615,10 +1606,10 @@ struct NeighbourKernels {
     {
         const dim3 g(blocks_for(t.nslots, TD));
         if (f.ncount)
-            hipLaunchKernelGGL((k_density<TD, U, kLists, kMasks, true>), g, dim3(TD), 0, st, p, cell, start, f,
+            hipLaunchKernelGGL((k_density<TD, U, kLists, kMasks, true, kRef>), g, dim3(TD), 0, st, p, cell, start, f,
                                nl, nl.masks, t, k);
         else
-            hipLaunchKernelGGL((k_density<TD, U, kLists, kMasks, false>), g, dim3(TD), 0, st, p, cell, start, f,
+            hipLaunchKernelGGL((k_density<TD, U, kLists, kMasks, false, kRef>), g, dim3(TD), 0, st, p, cell, start, f,
                                nl, nl.masks, t, k);
     }
     static void density(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
@@ -1638,10 +1629,10 @@ struct NeighbourKernels {
     {
         const dim3 g(blocks_for(t.nslots, T));
         if (ax)
-            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kQueue, kMode, true>), g, dim3(T), 0, st, p, cell, start, f,
+            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kQueue, kMode, true, kRef>), g, dim3(T), 0, st, p, cell, start, f,
                                nl, out, key_next, ax, ay, az, t, k);
         else
-            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kQueue, kMode, false>), g, dim3(T), 0, st, p, cell, start, f,
+            hipLaunchKernelGGL((k_forces_fused<T, U, C, kStall, kQueue, kMode, false, kRef>), g, dim3(T), 0, st, p, cell, start, f,
                                nl, out, key_next, ax, ay, az, t, k);
     }
     static void fused(hipStream_t st, const ParticleSoA &p, const uint32_t *cell, const uint32_t *start,
@@ -1804,10 +1795,18 @@ void launch_reorder(const ParticleSoA &src, const ParticleSoA &dst, const uint32
     hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256 * kReorderSlots)), dim3(256), 0, st, src, dst, perm, n);
 }
 
+// The reference order (kRef) is compiled in one shape, variant 12's.
+using RefKernels = NeighbourKernels<64, 8, 48, 0, true, true>;
+
 void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
-                    const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, hipStream_t st)
+                    const NeighbourList &nl, const SceneTable &t, const Consts &k, int variant, bool ref,
+                    hipStream_t st)
 {
     if (t.nslots == 0) return;
+    if (ref) {
+        RefKernels::density(st, p, cell, start, f, nl, t, k);
+        return;
+    }
     switch (variant) {
 #define SPH_DENSITY_CASE(i, t_, u, c, stl, q) \
     case i: NeighbourKernels<t_, u, c, stl, q>::density(st, p, cell, start, f, nl, t, k); break;
@@ -1819,11 +1818,15 @@ void launch_density(const ParticleSoA &p, const uint32_t *cell, const uint32_t *
 
 void launch_forces(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
                    const NeighbourList &nl, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
-                   hipStream_t st)
+                   bool ref, hipStream_t st)
 {
     if (t.nslots == 0) return;
-    hipLaunchKernelGGL(k_forces, dim3(blocks_for(t.nslots, kPhaseThreads)), dim3(kPhaseThreads), 0, st, p, cell,
-                       start, f, nl, ax, ay, az, t, k);
+    if (ref)
+        hipLaunchKernelGGL(k_forces<true>, dim3(blocks_for(t.nslots, kPhaseThreads)), dim3(kPhaseThreads), 0, st, p,
+                           cell, start, f, nl, ax, ay, az, t, k);
+    else
+        hipLaunchKernelGGL(k_forces<false>, dim3(blocks_for(t.nslots, kPhaseThreads)), dim3(kPhaseThreads), 0, st, p,
+                           cell, start, f, nl, ax, ay, az, t, k);
 }
 
 void launch_collide(const ParticleSoA &p, float *ax, float *ay, float *az, const SceneTable &t, const Consts &k,
@@ -1843,9 +1846,13 @@ void launch_integrate(const ParticleSoA &p, const float *ax, const float *ay, co
 
 void launch_forces_fused(const ParticleSoA &p, const uint32_t *cell, const uint32_t *start, const DensityFields &f,
                          const NeighbourList &nl, const ParticleSoA &out, uint32_t *key_next, float *ax, float *ay,
-                         float *az, const SceneTable &t, const Consts &k, int variant, hipStream_t st)
+                         float *az, const SceneTable &t, const Consts &k, int variant, bool ref, hipStream_t st)
 {
     if (t.nslots == 0) return;
+    if (ref) {
+        RefKernels::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k);
+        return;
+    }
     switch (variant) {
 #define SPH_FUSED_CASE(i, t_, u, c, stl, q) \
     case i: NeighbourKernels<t_, u, c, stl, q>::fused(st, p, cell, start, f, nl, out, key_next, ax, ay, az, t, k); break;

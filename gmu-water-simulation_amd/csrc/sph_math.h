@@ -49,22 +49,69 @@ __device__ __forceinline__ double next_double(double x, int dir)
     return __longlong_as_double(__double_as_longlong(x) + (long long) dir);
 }
 
-// Whether rounding the double x to float can change when x moves by up to 6 of
-// its ulps: only if x lies within 6 ulps of a float rounding boundary, the
+// Whether rounding the double x to float can change when x moves by up to 32 of
+// its ulps: only if x lies within 32 ulps of a float rounding boundary, the
 // midpoint of two adjacent floats, whose 29 low significand bits are 1 followed
 // by 28 zeros in x's binade (a boundary in the neighbouring binade is at least
-// 2^-25 x away). The window here is 16 ulps either side.
+// 2^-25 x away).
 //
-// The kernels use it to consult the pow tables only when the bits they keep can
-// depend on them. A power p replaced by its neighbour p' moves a product c*p by
-// at most 2 ulps before rounding (the relative spacing of doubles is within a
-// factor 2), so fl(c*p) by at most 4; a sum fl(D + fl(c*p)) with D >= 0 then
-// moves by at most 5.5 of its own ulps. Outside the window both choices give
-// the same float.
+// The kernels use it to compute a power cheaply and take the exact path (the
+// correctly rounded power, corrected by the pow tables) only when the float they
+// keep can depend on the difference. A power p' within k ulps of the library's
+// p moves a product c*p by at most 2k ulps before rounding (the relative spacing
+// of doubles is within a factor 2), so fl(c*p) by at most 2k + 1, and a sum
+// fl(D + fl(c*p)) with D >= 0 by at most 2k + 2 of its own ulps. The cheap cube
+// fl(fl(t*t)*t) is within 2 ulps of t^3 and pow(t, 3) within 1.5 (glibc: one
+// ulp from the correctly rounded cube at most, enumerated), so k <= 3.5 and the
+// sum moves by at most 9 ulps; the square hr*hr is correctly rounded, pow(hr, 2)
+// at most one ulp away. Outside the window every choice gives the same float.
 __device__ __forceinline__ bool near_float_boundary(double x)
 {
     const uint32_t lo = (uint32_t) __double_as_longlong(x);
-    return ((lo & 0x1FFFFFFFu) - (0x10000000u - 16u)) < 32u;
+    return ((lo & 0x1FFFFFFFu) - (0x10000000u - 32u)) < 64u;
+}
+
+// The library's pow(t, 3) for t = h^2 - r^2 of the float r^2 with bits r2_bits:
+// the correctly rounded cube, moved to its neighbour where the table lists r^2.
+__device__ __forceinline__ double pow3_lib(double t, uint32_t r2_bits, const uint32_t *tab, uint32_t n)
+{
+    double c = cube_cr(t);
+    const int d = pow_exception(tab, n, r2_bits);
+    return d ? next_double(c, d) : c;
+}
+
+// The library's pow(hr, 2) for hr = h - sqrt(r^2) of the float r^2 with bits r2_bits.
+__device__ __forceinline__ double pow2_lib(double hr, uint32_t r2_bits, const uint32_t *tab, uint32_t n)
+{
+    const double q = hr * hr;
+    const int d = pow_exception(tab, n, r2_bits);
+    return d ? next_double(q, d) : q;
+}
+
+// One term of updateDensityPressure (src/CCPUParticleSimulator.cpp:126): the
+// float density plus poly6 * pow(hsq - r^2, 3) (Wpoly6, :14) in double, rounded
+// to float. The cube is computed as fl(fl(t*t)*t); where the float result could
+// depend on its last bits (near_float_boundary), the sum is redone with the
+// library's cube (pow3_lib, table `tab3`).
+__device__ __forceinline__ float density_term(float density, float r2, double hsq, double poly6,
+                                              const uint32_t *tab3, uint32_t n3)
+{
+    const double t = hsq - (double) r2;
+    double sum = (double) density + poly6 * ((t * t) * t);
+    if (__builtin_expect(near_float_boundary(sum), 0))
+        sum = (double) density + poly6 * pow3_lib(t, __float_as_uint(r2), tab3, n3);
+    return (float) sum;
+}
+
+// (float)(spiky * pow(hr, 2)), the factor of WspikyGradient
+// (src/CCPUParticleSimulator.cpp:21) that meets a QVector3D: the correctly
+// rounded square, and the library's square (pow2_lib, table `tab2`) where the
+// float could depend on the difference.
+__device__ __forceinline__ float spiky_factor(double hr, float r2, double spiky, const uint32_t *tab2, uint32_t n2)
+{
+    double sg = spiky * (hr * hr);
+    if (__builtin_expect(near_float_boundary(sg), 0)) sg = spiky * pow2_lib(hr, __float_as_uint(r2), tab2, n2);
+    return (float) sg;
 }
 
 // sqrt(x), correctly rounded, for x the double of a finite non-negative float

@@ -216,9 +216,9 @@ sph_status step_noalloc(const sph_params *prm, const sph_device_state *st, uint3
         cur = 1 - cur;
         idc = 1 - idc;
         // density, then forces + walls + integrate + the next step's keys
-        launch_density(soa(cur, idc), key_s, cstart, f, nl, tab, k, kDefaultVariant, stream);
+        launch_density(soa(cur, idc), key_s, cstart, f, nl, tab, k, kDefaultVariant, false, stream);
         launch_forces_fused(soa(cur, idc), key_s, cstart, f, nl, soa(1 - cur, idc), key_h, nullptr, nullptr, nullptr,
-                            tab, k, kDefaultVariant, stream);
+                            tab, k, kDefaultVariant, false, stream);
         cur = 1 - cur;
         SPH_NOALLOC_TRY(check_launch("step launch"));
     }

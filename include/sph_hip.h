@@ -61,7 +61,9 @@ enum {
                                           z inner (:108-121); uploaded and emitted particles join the back of
                                           cell (0,0,0)'s bucket as addParticle does (src/CBaseParticleSimulator.cpp:
                                           67-74). Cells are then numbered z + y*Z + x*Y*Z (the reference's
-                                          traversal order) in sph_debug. Slower grid phase; same arithmetic. */
+                                          traversal order) in sph_debug. Slower grid phase; same arithmetic;
+                                          the neighbour kernels run in one shape (variant 12's) whatever
+                                          sph_set_variant chooses. */
 };
 
 /* sph_debug kinds. Per-particle arrays are in the current array order unless noted. */

@@ -152,6 +152,37 @@ __global__ __launch_bounds__(256) void k_pow_fingerprint(uint32_t lo, uint32_t c
     }
 }
 
+// The kernels' density term and spiky factor (cheap power, exact path near a
+// float rounding boundary) against the exact path everywhere, on n random cases:
+// r^2 a random float in [0, fl(h^2)], the density a random float in [300, 65536]
+// (the isolated-particle density is 328).
+__global__ __launch_bounds__(256) void k_pow_paths(uint64_t seed, uint64_t n, double hsq, double h, double poly6,
+                                                   double spiky, uint32_t hsq_bits, unsigned long long *bad,
+                                                   unsigned long long *first)
+{
+    uint32_t local = 0;
+    uint64_t fl = ~0ull;
+    for (uint64_t t = blockIdx.x * 256ull + threadIdx.x; t < n; t += (uint64_t) gridDim.x * 256ull) {
+        const uint64_t s = seed + t * 4ull * 0x9E3779B97F4A7C15ull;
+        const uint32_t rb = (uint32_t) (mix64(s + 1) % ((uint64_t) hsq_bits + 1));
+        const float r2 = __uint_as_float(rb);
+        const float dens = ldexpf(1.0f + (float) (uint32_t) (mix64(s + 2) & 0x7FFFFF) * 0x1p-23f,
+                                  8 + (int) (mix64(s + 3) % 8));   // [256, 65536)
+        const double tt = hsq - (double) r2;
+        const float fast = sph::density_term(dens, r2, hsq, poly6, sph::g_pow3_exc, sph::g_pow3_exc_count);
+        const float exact = (float) ((double) dens +
+                                     poly6 * sph::pow3_lib(tt, rb, sph::g_pow3_exc, sph::g_pow3_exc_count));
+        const double hr = h - sph::sqrt_of_float((double) r2);
+        const float sf = sph::spiky_factor(hr, r2, spiky, sph::g_pow2_exc, sph::g_pow2_exc_count);
+        const float se = (float) (spiky * sph::pow2_lib(hr, rb, sph::g_pow2_exc, sph::g_pow2_exc_count));
+        if (__float_as_uint(fast) != __float_as_uint(exact) || __float_as_uint(sf) != __float_as_uint(se)) {
+            ++local;
+            fl = fl < t ? fl : t;
+        }
+    }
+    report(local, fl, bad, first);
+}
+
 int run(void (*launch)(unsigned long long *, unsigned long long *), unsigned long long *bad_out,
         unsigned long long *first_out)
 {
@@ -173,6 +204,7 @@ int run(void (*launch)(unsigned long long *, unsigned long long *), unsigned lon
 
 uint32_t g_lo, g_count;
 uint64_t g_seed, g_n;
+double g_c[4];
 
 // glibc's pow, called through a pointer so that the compiler cannot replace pow(x, 2.0) by x * x.
 double (*volatile g_pow)(double, double) = std::pow;
@@ -277,6 +309,26 @@ int mc_pow_host(uint32_t lo, uint32_t hi, double hsq, double h, int threads, con
         out[3] = o[3] < out[3] ? o[3] : out[3];
     }
     return 0;
+}
+
+// Mismatches of the kernels' cheap-power paths against the exact ones on n random
+// cases (constants: hsq, h, poly6, spiky); 0 on success.
+int mc_pow_paths(uint64_t seed, uint64_t n, double hsq, double h, double poly6, double spiky,
+                 unsigned long long *bad, unsigned long long *first)
+{
+    g_seed = seed;
+    g_n = n;
+    g_c[0] = hsq;
+    g_c[1] = h;
+    g_c[2] = poly6;
+    g_c[3] = spiky;
+    return run([](unsigned long long *b, unsigned long long *f) {
+        float hf = (float) g_c[0];
+        uint32_t hb;
+        std::memcpy(&hb, &hf, 4);
+        hipLaunchKernelGGL(k_pow_paths, dim3(8192), dim3(256), 0, 0, g_seed, g_n, g_c[0], g_c[1], g_c[2], g_c[3], hb,
+                           b, f);
+    }, bad, first);
 }
 
 // Device fingerprint sums (out[0], out[1]) of the same results over lo..hi.

@@ -35,6 +35,8 @@ def mc():
     lib.mc_sqrt.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
     lib.mc_div.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
     lib.mc_pow_device.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.POINTER(C.c_ulonglong)]
+    lib.mc_pow_paths.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
     return lib
 
 
@@ -64,3 +66,15 @@ def test_library_powers_equal_host_pow_exhaustive(mc):
     assert mc.mc_pow_device(0, HSQ_BITS, hsq, float(np.float32(0.0457)), dev) == 0
     assert dev[0] == host[4], "device pow(h^2 - r^2, 3) differs from the host's pow"
     assert dev[1] == host[5], "device pow(h - r, 2) differs from the host's pow"
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_cheap_power_paths_equal_exact_paths(mc, orc, seed):
+    """density_term / spiky_factor (the cheap power, and the exact library power only
+    near a float rounding boundary) give bitwise the float of the exact path, on
+    2^28 random (density, r^2) cases per run."""
+    c = orc.constants()
+    bad, first = C.c_ulonglong(0), C.c_ulonglong(0)
+    assert mc.mc_pow_paths(seed, 1 << 28, c["hsq"], c["h"], c["poly6"], c["spiky"], C.byref(bad),
+                           C.byref(first)) == 0
+    assert bad.value == 0, "%d mismatches, first at case %d" % (bad.value, first.value)
