@@ -8,28 +8,26 @@
 // not derive from CGPUBaseParticleSimulator, which is bound to OpenCL
 // (include/CGPUBaseParticleSimulator.h:22-35).
 //
-// Data flow: the base class owns m_clParticles (id order) and the CParticle
-// render entities, which the GPU simulators keep in grid cell (0,0,0). The
-// device holds the state in cell-sorted structure-of-arrays buffers; integrate()
-// downloads by id into m_clParticles and refreshes the render entities, as
+// The device work is sph::GridSimulatorCore (sph_simulator_core.h, no Qt); this
+// class gives it the base class's state. The base class owns m_clParticles (id
+// order) and the CParticle render entities, which the GPU simulators keep in
+// grid cell (0,0,0). updateGrid() appends the particles generateParticles added
+// (FOUNTAIN) to the device; integrate() writes the device state back into
+// m_clParticles by id and refreshes the render entities, as
 // CGPUBaseParticleSimulator::integrate does (src/CGPUBaseParticleSimulator.cpp:76-97).
-// Particles that generateParticles appends (FOUNTAIN) are appended on the device
-// by the next updateGrid() instead of a full re-upload.
 #ifndef CHIPGRIDPARTICLESIMULATOR_H
 #define CHIPGRIDPARTICLESIMULATOR_H
 
-#include <string>
-#include <vector>
-
 #include "CBaseParticleSimulator.h"
-#include "sph_hip.h"
+#include "sph_simulator_core.h"
 
 class CHIPGridParticleSimulator : public CBaseParticleSimulator
 {
 public:
+    // flags: SPH_FLAG_* of the context, e.g. SPH_FLAG_REFERENCE_ORDER for sums in
+    // CCPUParticleSimulator's own order.
     explicit CHIPGridParticleSimulator(CScene *scene, float boxSize, SimulationScenario scenario = DAM_BREAK,
-                                       QObject *parent = nullptr);
-    ~CHIPGridParticleSimulator() override;
+                                       QObject *parent = nullptr, uint32_t flags = 0);
 
     QString getSelectedDevice() override;
     void setupScene() override;
@@ -44,17 +42,13 @@ protected:
     double integrate() override;
 
 private:
-    // Runs one phase; on failure reports like the OpenCL simulator does on a
-    // CLException (src/CGPUBaseParticleSimulator.cpp:28-37).
-    double phase(int which);
-    bool ok(sph_status status);
-    void appendNewParticles();
+    // After a core call: a failure is reported once, like a CLException
+    // (src/CGPUBaseParticleSimulator.cpp:28-37), and stops the timer.
+    double checked(double ms);
 
-    sph_ctx *m_ctx = nullptr;
-    uint32_t m_onDevice = 0;   // particles of m_clParticles already uploaded
-    bool m_failed = false;
-    std::string m_error;
-    std::vector<float> m_pos, m_vel;
+    sph::GridSimulatorCore m_core;
+    uint32_t m_flags;
+    bool m_reported = false;
 };
 
 #endif
