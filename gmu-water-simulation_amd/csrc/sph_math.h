@@ -162,6 +162,16 @@ __device__ __forceinline__ bool subnormal(float q)
     return __builtin_amdgcn_classf(q, 0x090);   // -subnormal, +subnormal
 }
 
+// Whether none of three quotients is zero or subnormal: their smallest
+// magnitude is at least the smallest normal float (NaNs drop out of the
+// minimum, and are neither zero nor subnormal). One minimum and one compare
+// before the per-quotient tests, which only run when this fails.
+__device__ __forceinline__ bool all_normal3(float q0, float q1, float q2)
+{
+    return __builtin_fminf(__builtin_fminf(__builtin_fabsf(q0), __builtin_fabsf(q1)), __builtin_fabsf(q2)) >=
+           0x1p-126f;
+}
+
 // a_c / b for b in (0, 1): |a/b| > |a| >= 2^-149 for a != 0, so a zero result
 // means a = 0 (exact); only subnormal results take the IEEE division.
 __device__ __forceinline__ void div3_small_divisor(float a0, float a1, float a2, float b, float &q0, float &q1,
@@ -171,7 +181,8 @@ __device__ __forceinline__ void div3_small_divisor(float a0, float a1, float a2,
     q0 = (float) ((double) a0 * rb);
     q1 = (float) ((double) a1 * rb);
     q2 = (float) ((double) a2 * rb);
-    if (__builtin_expect((int) subnormal(q0) | (int) subnormal(q1) | (int) subnormal(q2), 0)) {
+    if (__builtin_expect(!all_normal3(q0, q1, q2), 0) &&
+        ((int) subnormal(q0) | (int) subnormal(q1) | (int) subnormal(q2))) {
         q0 = a0 / b;
         q1 = a1 / b;
         q2 = a2 / b;
@@ -187,6 +198,7 @@ __device__ __forceinline__ void div3_rcp(float a0, float a1, float a2, double rb
     q0 = (float) ((double) a0 * rb);
     q1 = (float) ((double) a1 * rb);
     q2 = (float) ((double) a2 * rb);
+    if (__builtin_expect(all_normal3(q0, q1, q2), 1)) return;
     const bool t0 = zero_or_subnormal(q0) & (a0 != 0.0f);
     const bool t1 = zero_or_subnormal(q1) & (a1 != 0.0f);
     const bool t2 = zero_or_subnormal(q2) & (a2 != 0.0f);
