@@ -115,18 +115,18 @@ __device__ __forceinline__ float spiky_factor(double hr, float r2, double spiky,
 }
 
 // sqrt(x), correctly rounded, for x the double of a finite non-negative float
-// (the r^2 of an accepted pair, src/CCPUParticleSimulator.cpp:20). The
-// Goldschmidt refinement of v_rsq_f64 that the compiler emits for a double
-// sqrt, without its rescaling of arguments below 2^-767 (a positive float is at
-// least 2^-149) and with sqrt(0) = 0 selected. Checked against sqrt() for every
-// float in [0, h^2] on the device.
+// (the r^2 of an accepted pair, src/CCPUParticleSimulator.cpp:20): from the
+// hardware estimates g = x * rsq(x) and rsq(x) / 2, two Newton corrections
+// g += (x - g^2) * rsq(x) / 2, each residual exact by FMA, and sqrt(0) = 0
+// selected. Not correctly rounded for every double, but for every float in
+// [0, h^2]: checked against sqrt() for each of them on the device (a third
+// fewer FP64 operations than the compiler's Goldschmidt sequence for a double
+// sqrt, which this replaced after the same check).
 __device__ __forceinline__ double sqrt_of_float(double x)
 {
     const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, hh = y * 0.5;
-    const double r = __builtin_fma(-hh, g, 0.5);
-    g = __builtin_fma(g, r, g);
-    hh = __builtin_fma(hh, r, hh);
+    const double hh = y * 0.5;
+    double g = x * y;
     double d = __builtin_fma(-g, g, x);
     g = __builtin_fma(d, hh, g);
     d = __builtin_fma(-g, g, x);
