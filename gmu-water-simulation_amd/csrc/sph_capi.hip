@@ -500,8 +500,9 @@ sph_status rebuild_layout(sph_ctx *c)
             tl.start = a + k * (uint32_t) sph::kSortTile;
             tl.count = std::min<uint32_t>((uint32_t) sph::kSortTile, b - tl.start);
             tl.coff = seg.coff;
-            tl.hbase = first * (uint32_t) sph::kRadixBuckets + k;
-            tl.stride = count;
+            tl.first = first;
+            tl.ntiles = count;
+            tl.index = k;
             c->tiles_h.push_back(tl);
         }
         c->sort_bits = std::max(c->sort_bits, key_bits(seg.ncells + 1));
@@ -540,8 +541,9 @@ sph_status rebuild_layout(sph_ctx *c)
             tl.start = k * (uint32_t) sph::kSortTile;
             tl.count = std::min<uint32_t>((uint32_t) sph::kSortTile, c->nslots - tl.start);
             tl.coff = 0;
-            tl.hbase = k;
-            tl.stride = nt;
+            tl.first = 0;
+            tl.ntiles = nt;
+            tl.index = k;
             c->tiles_flat_h.push_back(tl);
         }
         if (nt > c->tiles_cap) return fail(SPH_ERR_CAPACITY, "sort tiles exceed their buffer");
@@ -634,19 +636,21 @@ sph_status run_emit(sph_ctx *c)
 }
 
 // Stable LSD radix sort of (keys, vals) over the given tiles into key_s / val_s
-// (vals == nullptr: the slot index), in passes of 8 bits up to `bits`.
+// (vals == nullptr: the slot index), in passes of sph::radix_plan's digits up to `bits`.
 void radix_sort(sph_ctx *c, const uint32_t *keys, const uint32_t *vals, const sph::SortTile *tiles, uint32_t ntiles,
                 int bits)
 {
-    const int passes = (bits + sph::kRadixBits - 1) / sph::kRadixBits;
+    int digit[8];
+    const int passes = sph::radix_plan(bits, digit);
     const uint32_t *kin = keys;
     const uint32_t *vin = vals;
+    int shift = 0;
     for (int p = 0; p < passes; ++p) {
         const bool to_s = ((passes - 1 - p) % 2) == 0;
         uint32_t *kout = to_s ? c->key_s : c->key_t;
         uint32_t *vout = to_s ? c->val_s : c->val_t;
-        sph::launch_radix_pass(kin, vin, kout, vout, tiles, ntiles, p * sph::kRadixBits, c->hist, c->partials,
-                               c->stream);
+        sph::launch_radix_pass(kin, vin, kout, vout, tiles, ntiles, shift, digit[p], c->hist, c->partials, c->stream);
+        shift += digit[p];
         kin = kout;
         vin = vout;
     }
@@ -1052,7 +1056,7 @@ sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
     SPH_TRY(dalloc(c, &c->val_t, P));
     SPH_TRY(dalloc(c, &c->val_s, P));
     c->tiles_cap = sph::max_sort_tiles(c->pool, max_segments);
-    const size_t hist_n = (size_t) sph::kRadixBuckets * c->tiles_cap;
+    const size_t hist_n = (size_t) sph::kRadixBucketsMax * c->tiles_cap;
     SPH_TRY(dalloc(c, &c->hist, hist_n));
     SPH_TRY(dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n)));
     SPH_TRY(dalloc(c, &c->tiles_d, c->tiles_cap));

@@ -62,8 +62,11 @@ __host__ __device__ inline size_t list_slot(const NeighbourList &nl, uint32_t i)
     return (size_t) (i >> 6) * nl.kmax * 64 + (i & 63u);
 }
 
+// Radix digits of 8 bits, or of 10 bits where that saves a pass (keys of 17-20
+// bits: two passes instead of three).
 constexpr int kRadixBits = 8;
-constexpr int kRadixBuckets = 1 << kRadixBits;
+constexpr int kRadixBitsWide = 10;
+constexpr int kRadixBucketsMax = 1 << kRadixBitsWide;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
@@ -100,26 +103,38 @@ void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, doubl
 
 // A tile of the radix sort: up to kSortTile slots [start, start + count) of one
 // segment (a scene or a free range) whose cells start at coff. The digits are
-// those of the segment-local key (key - coff), and the tile's digit counts go
-// to hist[hbase + d * stride], stride = the segment's tile count, so the
-// histograms are ordered segment, digit, tile: their exclusive scan places each
-// segment's items in the segment's own slots, sorted by local key, which is
-// the order of a sort of the global keys (segments never share a slot, and the
-// segments' cells come in slot order).
+// those of the segment-local key (key - coff). The segment's tiles are first ..
+// first + ntiles - 1 of the launch, this one `index` among them; in a pass of B
+// digits the tile's count of digit d goes to hist[first * B + d * ntiles +
+// index], so the histograms are ordered segment, digit, tile: their exclusive
+// scan places each segment's items in the segment's own slots, sorted by local
+// key, which is the order of a sort of the global keys (segments never share a
+// slot, and the segments' cells come in slot order).
 struct SortTile {
-    uint32_t start, count, coff, hbase, stride, pad[3];
+    uint32_t start, count, coff, first, ntiles, index, pad[2];
 };
+
+// Digit widths of a sort of `bits`-bit keys: all kRadixBits, or all
+// kRadixBitsWide where that takes fewer passes. Returns the pass count.
+inline int radix_plan(int bits, int *digit_bits)
+{
+    const int narrow = (bits + kRadixBits - 1) / kRadixBits, wide = (bits + kRadixBitsWide - 1) / kRadixBitsWide;
+    const int passes = wide < narrow ? wide : narrow, w = wide < narrow ? kRadixBitsWide : kRadixBits;
+    for (int p = 0; p < passes; ++p) digit_bits[p] = w;
+    return passes;
+}
 
 // Tiles of kSortTile slots over segments; returns the tile count (at most
 // sort_blocks(nslots) + the number of segments).
 inline uint32_t max_sort_tiles(uint32_t pool, uint32_t max_segments) { return sort_blocks(pool) + max_segments; }
 
-// One stable LSD radix pass over 8 bits of the segment-local keys starting at
-// `shift`. vals_in == nullptr means the identity (val = slot). hist holds
-// kRadixBuckets * ntiles entries, partials scan_blocks(kRadixBuckets * ntiles).
+// One stable LSD radix pass over `bits` (kRadixBits or kRadixBitsWide) bits of
+// the segment-local keys starting at `shift`. vals_in == nullptr means the
+// identity (val = slot). hist holds 2^bits * ntiles entries, partials
+// scan_blocks(2^bits * ntiles).
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
-                       uint32_t *vals_out, const SortTile *tiles, uint32_t ntiles, int shift, uint32_t *hist,
-                       uint32_t *partials, hipStream_t st);
+                       uint32_t *vals_out, const SortTile *tiles, uint32_t ntiles, int shift, int bits,
+                       uint32_t *hist, uint32_t *partials, hipStream_t st);
 
 // In-place exclusive scan of m values; partials holds scan_blocks(m) entries.
 void launch_exclusive_scan(uint32_t *data, uint32_t m, uint32_t *partials, hipStream_t st);

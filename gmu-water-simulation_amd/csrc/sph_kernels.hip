@@ -234,14 +234,16 @@ __global__ __launch_bounds__(256) void k_emit(const Emitter *__restrict__ em, Sc
     if (threadIdx.x == 0) sp->n = n0 + cnt;
 }
 
-// Digit histogram of one tile, stored digit-major: hist[d * nblocks + block].
+// Digit histogram of one tile (2^BITS digits), stored digit-major (SortTile).
+template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__restrict__ keys,
                                                               const SortTile *__restrict__ tiles, int shift,
                                                               uint32_t *__restrict__ hist)
 {
-    __shared__ uint32_t h[kRadixBuckets];
+    constexpr uint32_t B = 1u << BITS;
+    __shared__ uint32_t h[B];
     const int t = threadIdx.x;
-    for (int d = t; d < kRadixBuckets; d += kSortThreads) h[d] = 0;
+    for (uint32_t d = t; d < B; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const SortTile tl = tiles[blockIdx.x];
     // The keys arrive nearly sorted (the previous step's order), so neighbouring
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t q = (uint32_t) (k * kSortThreads + t);
         const bool valid = q < tl.count;
-        const uint32_t dig = valid ? ((kv[k] - tl.coff) >> shift) & (kRadixBuckets - 1) : kRadixBuckets;
+        const uint32_t dig = valid ? ((kv[k] - tl.coff) >> shift) & (B - 1) : B;
         const uint32_t prev = (uint32_t) __shfl_up((int) dig, 1, 64);
         const bool head = lane == 0 || dig != prev;
         const uint64_t heads = __ballot(head);
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__
         if (head && valid) atomicAdd(&h[dig], len);
     }
     __syncthreads();
-    for (int d = t; d < kRadixBuckets; d += kSortThreads) hist[tl.hbase + (uint32_t) d * tl.stride] = h[d];
+    for (uint32_t d = t; d < B; d += kSortThreads) hist[tl.first * B + d * tl.ntiles + tl.index] = h[d];
 }
 
 // Exclusive scan of a 4096-element tile held 4 per thread by a 1024-thread block:
@@ -358,69 +360,78 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data
     }
 }
 
-// Stable scatter of one LSD radix pass. The tile's items (slot order: round k
-// covers slots tile + k*256 .. +255) are all loaded up front, then placed round
-// by round: equal digits inside a wave are matched with 8 ballots and ranked by
-// popcount of the lower lanes; the first lane of each digit records the wave's
-// count in LDS; the thread of each digit turns the round's four wave counts into
-// absolute offsets (its running base, from the pass histogram, in a register);
-// every item stores at offset + rank. Two barriers a round; the count table is
-// double-buffered (the thread of a digit clears the other buffer's column), so
-// the kernel needs 8 KB of LDS.
+// Stable scatter of one LSD radix pass. Wave w of a tile owns its items
+// w*64*kSortItems .. (w+1)*64*kSortItems - 1 (slot order), all loaded up front.
+// The waves first count their digits into LDS (runs of equal digits in a round
+// add their length with one atomic); the tile's offset of each digit (the scanned
+// histogram) plus the counts of the waves before gives each wave its first
+// offset per digit. Then every wave places its items round by round on its own,
+// with no barrier: equal digits inside a round are matched with BITS ballots and
+// ranked by popcount of the lower lanes; all lanes of a digit read its running
+// offset, store at offset + rank, and the first of them advances the offset.
+// Three barriers a tile; 2^BITS * 16 bytes of LDS.
+template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, const SortTile *__restrict__ tiles, int shift,
     const uint32_t *__restrict__ hist)
 {
+    constexpr uint32_t B = 1u << BITS;
     constexpr int W = kSortThreads / 64;
-    static_assert(kRadixBuckets == kSortThreads, "one thread per digit");
-    __shared__ uint32_t off[2][W][kRadixBuckets];
+    __shared__ uint32_t base[W][B];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const SortTile tl = tiles[blockIdx.x];
-    uint32_t base = hist[tl.hbase + (uint32_t) t * tl.stride];
-#pragma unroll
-    for (int q = 0; q < W; ++q) off[0][q][t] = 0u;
+    for (uint32_t d = t; d < W * B; d += kSortThreads) (&base[0][0])[d] = 0u;
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t q0 = (uint32_t) (w * 64 * kSortItems + lane);
     uint32_t keys[kSortItems], vals[kSortItems];
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t q = (uint32_t) (k * kSortThreads + t), idx = tl.start + q;
+        const uint32_t q = q0 + 64u * k, idx = tl.start + q;
         keys[k] = q < tl.count ? keys_in[idx] : 0u;
         vals[k] = q < tl.count ? (vals_in ? vals_in[idx] : idx) : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const int bf = k & 1;
-        const bool valid = (uint32_t) (k * kSortThreads + t) < tl.count;
-        const uint32_t dig = ((keys[k] - tl.coff) >> shift) & (kRadixBuckets - 1);
+        const bool valid = q0 + 64u * k < tl.count;
+        const uint32_t dig = valid ? ((keys[k] - tl.coff) >> shift) & (B - 1) : B;
+        const uint32_t prev = (uint32_t) __shfl_up((int) dig, 1, 64);
+        const bool head = lane == 0 || dig != prev;
+        const uint64_t heads = __ballot(head);
+        const uint64_t after = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+        const uint32_t len = (after ? (uint32_t) __builtin_ctzll(after) : 64u) - (uint32_t) lane;
+        if (head && valid) atomicAdd(&base[w][dig], len);
+    }
+    __syncthreads();
+    for (uint32_t d = t; d < B; d += kSortThreads) {
+        uint32_t run = hist[tl.first * B + d * tl.ntiles + tl.index];
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const uint32_t c = base[q][d];
+            base[q][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    uint32_t *mine = base[w];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const bool valid = q0 + 64u * k < tl.count;
+        const uint32_t dig = ((keys[k] - tl.coff) >> shift) & (B - 1);
         uint64_t m = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const bool bit = (dig >> b) & 1u;
             const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t) __popcll(m & below);
-        if (valid && rank == 0) off[bf][w][dig] = (uint32_t) __popcll(m);
-        __syncthreads();
-        {
-            // the thread of digit t: wave counts -> absolute offsets; clear the other buffer
-            uint32_t run = base;
-#pragma unroll
-            for (int q = 0; q < W; ++q) {
-                const uint32_t c = off[bf][q][t];
-                off[bf][q][t] = run;
-                run += c;
-                off[bf ^ 1][q][t] = 0u;
-            }
-            base = run;
-        }
-        __syncthreads();
+        const uint32_t off = mine[dig];   // the wave's lanes read before its first lane writes
+        if (valid && rank == 0) mine[dig] = off + (uint32_t) __popcll(m);
         if (valid) {
-            const uint32_t pos = off[bf][w][dig] + rank;
-            keys_out[pos] = keys[k];
-            vals_out[pos] = vals[k];
+            keys_out[off + rank] = keys[k];
+            vals_out[off + rank] = vals[k];
         }
     }
 }
@@ -1701,21 +1712,30 @@ void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, doubl
 }
 
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out,
-                       const SortTile *tiles, uint32_t ntiles, int shift, uint32_t *hist, uint32_t *partials,
+                       const SortTile *tiles, uint32_t ntiles, int shift, int bits, uint32_t *hist, uint32_t *partials,
                        hipStream_t st)
 {
     if (ntiles == 0) return;
-    const uint32_t m = ntiles * (uint32_t) kRadixBuckets;
+    const uint32_t m = ntiles << bits;
     const uint32_t sb = scan_blocks(m);
-    hipLaunchKernelGGL(k_radix_count, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, tiles, shift, hist);
+    if (bits == kRadixBitsWide)
+        hipLaunchKernelGGL(k_radix_count<kRadixBitsWide>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, tiles,
+                           shift, hist);
+    else
+        hipLaunchKernelGGL(k_radix_count<kRadixBits>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, tiles, shift,
+                           hist);
     if (sb == 1) {
         hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, hist, m);
     } else {
         hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(1024), 0, st, (const uint32_t *) hist, m, partials);
         hipLaunchKernelGGL(k_scan_apply, dim3(sb), dim3(1024), 0, st, hist, m, (const uint32_t *) partials);
     }
-    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in, keys_out, vals_out,
-                       tiles, shift, (const uint32_t *) hist);
+    if (bits == kRadixBitsWide)
+        hipLaunchKernelGGL(k_radix_scatter<kRadixBitsWide>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in,
+                           keys_out, vals_out, tiles, shift, (const uint32_t *) hist);
+    else
+        hipLaunchKernelGGL(k_radix_scatter<kRadixBits>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in,
+                           keys_out, vals_out, tiles, shift, (const uint32_t *) hist);
 }
 
 void launch_exclusive_scan(uint32_t *data, uint32_t m, uint32_t *partials, hipStream_t st)

@@ -43,7 +43,7 @@ using namespace sph;
 constexpr uint32_t kMaxN = SPH_NOALLOC_MAX_PARTICLES;
 constexpr uint32_t kMaxGridCells = SPH_NOALLOC_MAX_CELLS;
 constexpr uint32_t kMaxTiles = (kMaxN + kSortTile - 1) / kSortTile;
-constexpr uint32_t kHistEntries = kRadixBuckets * kMaxTiles;
+constexpr uint32_t kHistEntries = (1u << kRadixBits) * kMaxTiles;   // 8-bit passes only
 static_assert(kHistEntries <= kScanTile, "the histograms of one call are scanned by one block");
 
 constexpr size_t align256(size_t v) { return (v + 255) / 256 * 256; }
@@ -106,8 +106,9 @@ __global__ __launch_bounds__(64) void k_noalloc_setup(Scene s, uint32_t ntiles, 
         tl.start = k * (uint32_t) kSortTile;
         tl.count = min((uint32_t) kSortTile, s.n - tl.start);
         tl.coff = 0;
-        tl.hbase = k;
-        tl.stride = ntiles;
+        tl.first = 0;
+        tl.ntiles = ntiles;
+        tl.index = k;
         tiles[k] = tl;
     }
 }
@@ -207,7 +208,8 @@ sph_status step_noalloc(const sph_params *prm, const sph_device_state *st, uint3
         for (int p = 0; p < passes; ++p) {
             const bool to_s = ((passes - 1 - p) % 2) == 0;
             uint32_t *kout = to_s ? key_s : key_t, *vout = to_s ? val_s : val_t;
-            launch_radix_pass(kin, vin, kout, vout, tiles_d, ntiles, p * kRadixBits, hist, partials, stream);
+            launch_radix_pass(kin, vin, kout, vout, tiles_d, ntiles, p * kRadixBits, kRadixBits, hist, partials,
+                              stream);
             kin = kout;
             vin = vout;
         }
