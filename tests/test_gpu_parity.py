@@ -357,6 +357,32 @@ def test_full_size_one_step(hip, orc):
     assert_bits_equal(p, st["pos"], "positions 1M")
 
 
+@pytest.mark.parametrize("res", [30, 102, 257])
+def test_sort_across_key_widths(hip, orc, res):
+    """The grid phase's stable sort of (cell, slot) against numpy's stable argsort
+    of the oracle's cells, for grids whose keys take two 8-bit passes (30^3 cells,
+    15-bit keys), three 8-bit passes (102^3, 21 bits) and three 10-bit passes
+    (257^3, 25 bits); 200,000 particles uniform over the box, in random order."""
+    H = hip.scenes.H
+    box = float(np.float32((res - 0.5) * float(H)))
+    n = 200_000
+    rng = np.random.default_rng(res)
+    pos = ((rng.random((n, 3)) - 0.5) * box).astype(np.float32)
+    print("grid %d^3, box %.4f m" % (res, box), flush=True)
+    sim = gpu_sim(hip, box, pos, diag=False)
+    (rx, ry, rz), ncells = sim.grid_dims()
+    assert (rx, ry, rz) == (res, res, res)
+    sim.updateGrid()
+    pre = sim.debug(hip._lib.DEBUG_CELL_PRESORT)
+    np.testing.assert_array_equal(pre, orc.hash_positions(pos, box, res))
+    want = np.argsort(pre, kind="stable").astype(np.uint32)
+    np.testing.assert_array_equal(sim.debug(hip._lib.DEBUG_PERM), want)
+    np.testing.assert_array_equal(sim.debug(hip._lib.DEBUG_CELL), pre[want])
+    start = sim.debug(hip._lib.DEBUG_CELL_START)
+    np.testing.assert_array_equal(start, np.searchsorted(pre[want], np.arange(ncells + 1), side="left").astype(np.uint32))
+    sim.close()
+
+
 def test_full_workloads_stay_whole_finite_and_in_the_box(hip):
     """The benchmark workloads at full size and length: the 1M dam break for 1,000
     steps and the fountain to 2,000 steps (524,288 particles, graph replay). Every
