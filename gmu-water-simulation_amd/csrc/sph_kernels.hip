@@ -437,11 +437,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 }
 
 // start[c] = the first sorted slot whose key is >= c (lower bound), for every
-// c in [0, ncells]. Slot i (0 <= i <= n, slot n standing for the end) writes i
-// into every cell in (keys[i - 1], keys[i]] (from cell 0 for i = 0, up to ncells
-// for i = n), so each cell is written once and nothing is searched. A lane whose
-// range is longer than kFillShort cells (empty regions of a scene, the cells
-// after its last particle) has it filled by the whole wave, 64 cells a store.
+// c in [0, ncells]. Slot i (0 <= i < n) writes i into every cell in
+// (keys[i - 1], keys[i]] (from cell 0 for i = 0), so each cell is written once
+// and nothing is searched. A lane whose range is longer than kFillShort cells
+// (empty regions of a scene) has it filled by the whole wave, 64 cells a store.
+// The cells after the last key get n from every thread of the grid, a few
+// each: that range is the largest (the empty top of a single scene).
 constexpr uint32_t kFillShort = 8;
 
 __global__ __launch_bounds__(256) void k_cell_start(const uint32_t *__restrict__ keys, uint32_t n,
@@ -449,9 +450,9 @@ __global__ __launch_bounds__(256) void k_cell_start(const uint32_t *__restrict__
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x, lane = threadIdx.x & 63u;
     uint32_t lo = 1, len = 0;   // cells [lo, lo + len) get i
-    if (i <= n) {
+    if (i < n) {
         lo = i == 0 ? 0u : keys[i - 1] + 1u;
-        const uint32_t hi = min(i < n ? keys[i] : ncells, ncells);   // keys are cells <= ncells
+        const uint32_t hi = min(keys[i], ncells);   // keys are cells <= ncells
         len = lo <= hi ? hi - lo + 1u : 0u;
     }
     if (len <= kFillShort) {
@@ -465,6 +466,10 @@ __global__ __launch_bounds__(256) void k_cell_start(const uint32_t *__restrict__
         const uint32_t li = (uint32_t) __shfl((int) i, (int) l, 64);
         for (uint64_t q = lane; q < llen; q += 64u) start[llo + (uint32_t) q] = li;
     }
+    // the tail: cells (keys[n - 1], ncells], all of them for n = 0
+    const uint32_t tail = n == 0 ? 0u : min(keys[n - 1], ncells) + 1u;
+    const uint32_t threads = gridDim.x * 256u;
+    for (uint32_t c = tail + i; c <= ncells && c >= tail; c += threads) start[c] = n;
 }
 
 // Each thread moves kReorderSlots slots (blockDim apart), their loads in flight
