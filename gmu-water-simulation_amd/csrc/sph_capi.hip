@@ -1058,7 +1058,7 @@ sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
     c->tiles_cap = sph::max_sort_tiles(c->pool, max_segments);
     const size_t hist_n = (size_t) sph::kRadixBucketsMax * c->tiles_cap;
     SPH_TRY(dalloc(c, &c->hist, hist_n));
-    SPH_TRY(dalloc(c, &c->partials, sph::scan_blocks((uint32_t) hist_n)));
+    SPH_TRY(dalloc(c, &c->partials, 2 * ((size_t) sph::scan_blocks((uint32_t) hist_n) + 1)));
     SPH_TRY(dalloc(c, &c->tiles_d, c->tiles_cap));
     if (c->ref_order()) {
         SPH_TRY(dalloc(c, &c->ocell, P));

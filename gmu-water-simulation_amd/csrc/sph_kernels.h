@@ -130,8 +130,9 @@ inline uint32_t max_sort_tiles(uint32_t pool, uint32_t max_segments) { return so
 
 // One stable LSD radix pass over `bits` (kRadixBits or kRadixBitsWide) bits of
 // the segment-local keys starting at `shift`. vals_in == nullptr means the
-// identity (val = slot). hist holds 2^bits * ntiles entries, partials
-// scan_blocks(2^bits * ntiles).
+// identity (val = slot). hist holds 2^bits * ntiles entries; partials, 8-byte
+// aligned, holds the single-pass scan's state: 2 * (scan_blocks(2^bits * ntiles)
+// + 1) 32-bit words when that is more than one scan block (none otherwise).
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
                        uint32_t *vals_out, const SortTile *tiles, uint32_t ntiles, int shift, int bits,
                        uint32_t *hist, uint32_t *partials, hipStream_t st);

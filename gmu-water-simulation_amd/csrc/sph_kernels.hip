@@ -238,11 +238,16 @@ __global__ __launch_bounds__(256) void k_emit(const Emitter *__restrict__ em, Sc
 template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *__restrict__ keys,
                                                               const SortTile *__restrict__ tiles, int shift,
-                                                              uint32_t *__restrict__ hist)
+                                                              uint32_t *__restrict__ hist,
+                                                              unsigned long long *__restrict__ scan_state,
+                                                              uint32_t nscan_state)
 {
     constexpr uint32_t B = 1u << BITS;
     __shared__ uint32_t h[B];
     const int t = threadIdx.x;
+    // the next launch's single-pass scan starts from a zero ticket counter and no published tiles
+    if (blockIdx.x == 0)
+        for (uint32_t q = t; q < nscan_state; q += kSortThreads) scan_state[q] = 0ull;
     for (uint32_t d = t; d < B; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const SortTile tl = tiles[blockIdx.x];
@@ -353,6 +358,80 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data
     const uint32_t idx = blockIdx.x * (uint32_t) kScanTile + (uint32_t) threadIdx.x * 4u;
     uint32_t v[4], total;
     uint32_t run = carry + block_exclusive_scan(load4(data, idx, m, v), wsum, total);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (idx + q < m) data[idx + q] = run;
+        run += v[q];
+    }
+}
+
+// Single-pass exclusive scan of m values in tiles of kScanTile, with decoupled
+// look-back: each block takes the next tile by ticket (so a tile only waits on
+// tiles whose blocks have started), scans it, publishes its total, then its
+// first wave reads the published words of up to 64 predecessors at a time,
+// back to the nearest one that holds an inclusive prefix (tile 0 publishes one
+// at once), publishes its own inclusive prefix and writes its values. state[0]
+// is the ticket counter, state[1 + k] tile k's word: kind in the high half (0
+// nothing yet, 1 total, 2 inclusive prefix), value in the low half, so one
+// relaxed 64-bit atomic carries both and no other memory is shared between the
+// blocks; the radix count kernel zeroes the words before every pass. A wait
+// longer than kScanSpinLimit polls gives up (leaving the prefix short) instead
+// of hanging; that never happens while every started block runs to its end.
+constexpr uint32_t kScanSpinLimit = 1u << 22;
+
+__global__ __launch_bounds__(1024) void k_scan_chained(uint32_t *__restrict__ data, uint32_t m,
+                                                       unsigned long long *__restrict__ state)
+{
+    __shared__ uint32_t wsum[17];
+    __shared__ uint32_t s_tile, s_prefix;
+    if (threadIdx.x == 0) s_tile = (uint32_t) atomicAdd(&state[0], 1ull);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    unsigned long long *word = state + 1;
+    const uint32_t idx = tile * (uint32_t) kScanTile + (uint32_t) threadIdx.x * 4u;
+    uint32_t v[4], total;
+    const uint32_t local = block_exclusive_scan(load4(data, idx, m, v), wsum, total);
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&word[0], (2ull << 32) | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&word[tile], (1ull << 32) | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // window: predecessors top - lane, top = the nearest one not yet summed
+            int top = (int) tile - 1;
+            uint32_t polls = 0;
+            while (top >= 0) {
+                const int k = top - lane;
+                const unsigned long long w =
+                    k >= 0 ? __hip_atomic_load(&word[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ull << 32);
+                const uint32_t kind = (uint32_t) (w >> 32);
+                const unsigned long long incl = __ballot(kind == 2), none = __ballot(kind == 0);
+                // lanes up to the first inclusive prefix (all 64 when there is none in the window)
+                const int stop = incl ? __builtin_ctzll(incl) : 63;
+                const unsigned long long need = stop == 63 ? ~0ull : ((2ull << stop) - 1ull);
+                if (none & need) {   // a predecessor in the window has published nothing yet
+                    if (++polls >= kScanSpinLimit) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                uint32_t x = lane <= stop ? (uint32_t) w : 0u;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+                prefix += x;
+                if (incl) break;
+                top -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&word[tile], (2ull << 32) | (prefix + total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + local;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (idx + q < m) data[idx + q] = run;
@@ -1723,18 +1802,19 @@ void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_
     if (ntiles == 0) return;
     const uint32_t m = ntiles << bits;
     const uint32_t sb = scan_blocks(m);
+    // more than one scan tile: one single-pass scan, its state in `partials` (zeroed by the count kernel)
+    unsigned long long *state = sb > 1 ? reinterpret_cast<unsigned long long *>(partials) : nullptr;
+    const uint32_t nstate = sb > 1 ? sb + 1 : 0u;
     if (bits == kRadixBitsWide)
         hipLaunchKernelGGL(k_radix_count<kRadixBitsWide>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, tiles,
-                           shift, hist);
+                           shift, hist, state, nstate);
     else
         hipLaunchKernelGGL(k_radix_count<kRadixBits>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, tiles, shift,
-                           hist);
-    if (sb == 1) {
+                           hist, state, nstate);
+    if (sb == 1)
         hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, hist, m);
-    } else {
-        hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(1024), 0, st, (const uint32_t *) hist, m, partials);
-        hipLaunchKernelGGL(k_scan_apply, dim3(sb), dim3(1024), 0, st, hist, m, (const uint32_t *) partials);
-    }
+    else
+        hipLaunchKernelGGL(k_scan_chained, dim3(sb), dim3(1024), 0, st, hist, m, state);
     if (bits == kRadixBitsWide)
         hipLaunchKernelGGL(k_radix_scatter<kRadixBitsWide>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in,
                            keys_out, vals_out, tiles, shift, (const uint32_t *) hist);
