@@ -131,6 +131,39 @@ def test_batch_vs_oracle(hip, orc):
             o.close()
 
 
+def test_tiny_and_empty_scenes(hip, orc):
+    """Scenes of 0, 1, 2 and 3 particles beside a 4,096-particle one, stepped
+    together eagerly and from graphs: every live scene bitwise the oracle after
+    12 steps, the empty one still empty (its segment sorts to dead keys only)."""
+    H = np.float32(0.0457)
+    lattice = hip.scenes.lattice(16, 16, 16, 0.4, jitter_h=0.01, seed=9)
+    tiny = {1: np.array([[0.01, -0.15, 0.02]], np.float32),
+            2: np.array([[0.0, 0.0, 0.0], [0.3 * H, 0.2 * H, -0.1 * H]], np.float32),
+            3: np.array([[0.05, 0.1, -0.05], [0.05 + 0.5 * H, 0.1, -0.05], [0.05, 0.1 - 0.7 * H, -0.05 + 0.4 * H]],
+                        np.float32)}
+    scenes = [(0.4, 4), (0.3, 4), (0.2, 4), (0.25, 4), (0.4, len(lattice))]
+    content = [None, tiny[1], tiny[2], tiny[3], lattice]
+    for graphs in (False, True):
+        with hip.SceneBatch(scenes) as b:
+            for i, pos in enumerate(content):
+                if pos is not None:
+                    b.upload(i, pos)
+            b.set_graphs(graphs)
+            b.step(12)
+            assert b.count(0) == 0
+            for i, pos in enumerate(content):
+                if pos is None:
+                    continue
+                o = orc.OracleSim(scenes[i][0], len(pos))
+                o.add_particles(pos)
+                o.set_order_mode(orc.ORDER_CANONICAL)
+                o.step(12)
+                p, v, _ = b.download(i)
+                assert_bits_equal(p, o.state()["pos"], "scene %d (graphs %d) positions vs oracle" % (i, graphs))
+                assert_bits_equal(v, o.state()["vel"], "scene %d (graphs %d) velocities vs oracle" % (i, graphs))
+                o.close()
+
+
 def test_graph_replays_equal_eager(hip):
     """Graph replays with eager work (downloads, syncs) between them, every
     replay checked against the eager trajectory."""
