@@ -357,14 +357,16 @@ def test_full_size_one_step(hip, orc):
     assert_bits_equal(p, st["pos"], "positions 1M")
 
 
-@pytest.mark.parametrize("res,n", [(30, 200_000), (102, 200_000), (257, 200_000), (257, 1_200_000)])
+@pytest.mark.parametrize("res,n", [(30, 200_000), (102, 200_000), (257, 200_000), (257, 1_200_000),
+                                   (257, 8_000_000)])
 def test_sort_across_key_widths(hip, orc, res, n):
     """The grid phase's stable sort of (cell, slot) against numpy's stable argsort
     of the oracle's cells, for grids whose keys take two 8-bit passes (30^3 cells,
     15-bit keys), three 8-bit passes (102^3, 21 bits) and three 10-bit passes
     (257^3, 25 bits); n particles uniform over the box, in random order. With
     1,200,000 particles the histograms of a 10-bit pass fill 74 scan tiles, more
-    than the 64 predecessors one step of the scan's look-back reads."""
+    than the 64 predecessors one step of the scan's look-back reads; with
+    8,000,000 they fill 489."""
     H = hip.scenes.H
     box = float(np.float32((res - 0.5) * float(H)))
     rng = np.random.default_rng(res + n)
