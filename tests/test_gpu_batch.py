@@ -164,6 +164,33 @@ def test_tiny_and_empty_scenes(hip, orc):
                 o.close()
 
 
+def test_many_tiny_scenes(hip, orc):
+    """1,024 scenes of 1-40 particles (boxes 0.1-0.3 m), one sort tile each: the
+    histograms of every pass span 1,024 segments (a multi-tile scan), and every
+    scene after 6 steps is bitwise the oracle's."""
+    rng = np.random.default_rng(77)
+    scenes, content = [], []
+    for k in range(1024):
+        box = float(np.float32(rng.uniform(0.1, 0.3)))
+        n = int(rng.integers(1, 41))
+        pos = ((rng.random((n, 3)) - 0.5) * np.float32(box * 0.8)).astype(np.float32)
+        scenes.append((box, n))
+        content.append(pos)
+    with hip.SceneBatch(scenes) as b:
+        for i, pos in enumerate(content):
+            b.upload(i, pos)
+        b.step(6)
+        for i in range(0, 1024, 7):
+            o = orc.OracleSim(scenes[i][0], scenes[i][1])
+            o.add_particles(content[i])
+            o.set_order_mode(orc.ORDER_CANONICAL)
+            o.step(6)
+            p, v, _ = b.download(i)
+            assert_bits_equal(p, o.state()["pos"], "scene %d positions vs oracle" % i)
+            assert_bits_equal(v, o.state()["vel"], "scene %d velocities vs oracle" % i)
+            o.close()
+
+
 def test_graph_replays_equal_eager(hip):
     """Graph replays with eager work (downloads, syncs) between them, every
     replay checked against the eager trajectory."""
