@@ -1093,8 +1093,18 @@ __global__ __launch_bounds__(T) void k_density(ParticleSoA p, const uint32_t *__
         }
         return ok;
     };
+    // the terms of two pairs are formed before their in-order adds, so that the
+    // loads and FP64 work of one pair overlap the other's
     auto flush = [&]() {
-        for (int q = 0; q < cnt; ++q) density = density_add(density, my[q * T], k);
+        int q = 0;
+        for (; q + 1 < cnt; q += 2) {
+            const float ra = my[q * T], rb = my[(q + 1) * T];
+            const double ta = k.hsq - (double) ra, tb = k.hsq - (double) rb;
+            const double pa = density_poly(ta, k.poly6), pb = density_poly(tb, k.poly6);
+            density = density_accumulate(density, pa, ta, ra, k.poly6, g_pow3_exc, g_pow3_exc_count);
+            density = density_accumulate(density, pb, tb, rb, k.poly6, g_pow3_exc, g_pow3_exc_count);
+        }
+        if (q < cnt) density = density_add(density, my[q * T], k);
     };
     if constexpr (kMasks)
         walk_neighbours<kRef, U, false, kCap>(s, k, start, bv, x, y, z, xi, yi, zi, cnt, test, flush, rec_masks);

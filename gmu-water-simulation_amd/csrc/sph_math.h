@@ -93,14 +93,29 @@ __device__ __forceinline__ double pow2_lib(double hr, uint32_t r2_bits, const ui
 // to float. The cube is computed as fl(fl(t*t)*t); where the float result could
 // depend on its last bits (near_float_boundary), the sum is redone with the
 // library's cube (pow3_lib, table `tab3`).
+//
+// Split in two so that a caller can compute the terms of several pairs before
+// adding them in order: density_poly is the part that does not depend on the
+// running density, density_accumulate the add and its rounding.
+__device__ __forceinline__ double density_poly(double t, double poly6)
+{
+    return poly6 * ((t * t) * t);
+}
+
+__device__ __forceinline__ float density_accumulate(float density, double poly, double t, float r2, double poly6,
+                                                    const uint32_t *tab3, uint32_t n3)
+{
+    double sum = (double) density + poly;
+    if (__builtin_expect(near_float_boundary(sum), 0))
+        sum = (double) density + poly6 * pow3_lib(t, __float_as_uint(r2), tab3, n3);
+    return (float) sum;
+}
+
 __device__ __forceinline__ float density_term(float density, float r2, double hsq, double poly6,
                                               const uint32_t *tab3, uint32_t n3)
 {
     const double t = hsq - (double) r2;
-    double sum = (double) density + poly6 * ((t * t) * t);
-    if (__builtin_expect(near_float_boundary(sum), 0))
-        sum = (double) density + poly6 * pow3_lib(t, __float_as_uint(r2), tab3, n3);
-    return (float) sum;
+    return density_accumulate(density, density_poly(t, poly6), t, r2, poly6, tab3, n3);
 }
 
 // (float)(spiky * pow(hr, 2)), the factor of WspikyGradient
