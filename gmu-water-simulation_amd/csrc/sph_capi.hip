@@ -135,6 +135,7 @@ struct sph_ctx {
     int next_phase = SPH_PHASE_GRID;
     uint64_t steps_done = 0;
     long fault_at_step = -1;
+    uint32_t scan_poll_limit = sph::kScanPollLimit;   // SPH_SCAN_POLL_LIMIT (tests)
 
     // graphs: one executable step per (cur, id_cur) parity
     bool use_graphs = false;
@@ -649,7 +650,8 @@ void radix_sort(sph_ctx *c, const uint32_t *keys, const uint32_t *vals, const sp
         const bool to_s = ((passes - 1 - p) % 2) == 0;
         uint32_t *kout = to_s ? c->key_s : c->key_t;
         uint32_t *vout = to_s ? c->val_s : c->val_t;
-        sph::launch_radix_pass(kin, vin, kout, vout, tiles, ntiles, shift, digit[p], c->hist, c->partials, c->stream);
+        sph::launch_radix_pass(kin, vin, kout, vout, tiles, ntiles, shift, digit[p], c->hist, c->partials, c->stream,
+                               c->scan_poll_limit);
         shift += digit[p];
         kin = kout;
         vin = vout;
@@ -1058,7 +1060,7 @@ sph_status alloc_buffers(sph_ctx *c, uint32_t max_segments)
     c->tiles_cap = sph::max_sort_tiles(c->pool, max_segments);
     const size_t hist_n = (size_t) sph::kRadixBucketsMax * c->tiles_cap;
     SPH_TRY(dalloc(c, &c->hist, hist_n));
-    SPH_TRY(dalloc(c, &c->partials, 2 * ((size_t) sph::scan_blocks((uint32_t) hist_n) + 1)));
+    SPH_TRY(dalloc(c, &c->partials, hist_n + 2 * ((size_t) sph::scan_blocks((uint32_t) hist_n) + 1)));
     SPH_TRY(dalloc(c, &c->tiles_d, c->tiles_cap));
     if (c->ref_order()) {
         SPH_TRY(dalloc(c, &c->ocell, P));
@@ -1138,6 +1140,7 @@ sph_status create_ctx(uint32_t pool, uint32_t max_scenes, uint32_t flags, uint32
         c->scenes[i].s.order = c->ref_order() ? sph::kOrderReference : sph::kOrderCanonical;
     }
     if (const char *f = std::getenv("SPH_FAULT_AT_STEP")) c->fault_at_step = std::atol(f);
+    if (const char *f = std::getenv("SPH_SCAN_POLL_LIMIT")) c->scan_poll_limit = (uint32_t) std::strtoul(f, nullptr, 10);
 
     hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (he != hipSuccess) {

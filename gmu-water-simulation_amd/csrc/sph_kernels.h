@@ -131,14 +131,20 @@ inline int radix_plan(int bits, int *digit_bits)
 // sort_blocks(nslots) + the number of segments).
 inline uint32_t max_sort_tiles(uint32_t pool, uint32_t max_segments) { return sort_blocks(pool) + max_segments; }
 
+// Polls of the histogram scan's look-back before a block adds up its
+// predecessors' raw counts itself (the result is the same either way).
+constexpr uint32_t kScanPollLimit = 1u << 22;
+
 // One stable LSD radix pass over `bits` (kRadixBits or kRadixBitsWide) bits of
 // the segment-local keys starting at `shift`. vals_in == nullptr means the
-// identity (val = slot). hist holds 2^bits * ntiles entries; partials, 8-byte
-// aligned, holds the single-pass scan's state: 2 * (scan_blocks(2^bits * ntiles)
-// + 1) 32-bit words when that is more than one scan block (none otherwise).
+// identity (val = slot). hist holds m = 2^bits * ntiles entries. When that is
+// more than one scan block, partials (8-byte aligned) receives the scanned
+// offsets (m words) followed by the single-pass scan's state (2 *
+// (scan_blocks(m) + 1) words); otherwise it is not used. poll_limit: see
+// kScanPollLimit (0: no look-back).
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
                        uint32_t *vals_out, const SortTile *tiles, uint32_t ntiles, int shift, int bits,
-                       uint32_t *hist, uint32_t *partials, hipStream_t st);
+                       uint32_t *hist, uint32_t *partials, hipStream_t st, uint32_t poll_limit = kScanPollLimit);
 
 // In-place exclusive scan of m values; partials holds scan_blocks(m) entries.
 void launch_exclusive_scan(uint32_t *data, uint32_t m, uint32_t *partials, hipStream_t st);

@@ -365,22 +365,22 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t *__restrict__ data
     }
 }
 
-// Single-pass exclusive scan of m values in tiles of kScanTile, with decoupled
-// look-back: each block takes the next tile by ticket (so a tile only waits on
-// tiles whose blocks have started), scans it, publishes its total, then its
-// first wave reads the published words of up to 64 predecessors at a time,
-// back to the nearest one that holds an inclusive prefix (tile 0 publishes one
-// at once), publishes its own inclusive prefix and writes its values. state[0]
-// is the ticket counter, state[1 + k] tile k's word: kind in the high half (0
-// nothing yet, 1 total, 2 inclusive prefix), value in the low half, so one
-// relaxed 64-bit atomic carries both and no other memory is shared between the
-// blocks; the radix count kernel zeroes the words before every pass. A wait
-// longer than kScanSpinLimit polls gives up (leaving the prefix short) instead
-// of hanging; that never happens while every started block runs to its end.
-constexpr uint32_t kScanSpinLimit = 1u << 22;
-
-__global__ __launch_bounds__(1024) void k_scan_chained(uint32_t *__restrict__ data, uint32_t m,
-                                                       unsigned long long *__restrict__ state)
+// Single-pass exclusive scan of the m values `in` into `out`, in tiles of
+// kScanTile, with decoupled look-back: each block takes the next tile by ticket
+// (so a tile only waits on tiles whose blocks have started), scans it, publishes
+// its total, then its first wave reads the published words of up to 64
+// predecessors at a time, back to the nearest one that holds an inclusive prefix
+// (tile 0 publishes one at once), publishes its own inclusive prefix and writes
+// its values. state[0] is the ticket counter, state[1 + k] tile k's word: kind in
+// the high half (0 nothing yet, 1 total, 2 inclusive prefix), value in the low
+// half, so one relaxed 64-bit atomic carries both and no other memory is shared
+// between the blocks; the radix count kernel zeroes the words before every pass.
+// The wait is bounded: a block that has polled poll_limit times (0: at once)
+// adds up the raw values of the predecessors it has not summed yet from `in`,
+// which no block writes, so the result is the same either way.
+__global__ __launch_bounds__(1024) void k_scan_chained(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                       uint32_t m, unsigned long long *__restrict__ state,
+                                                       uint32_t poll_limit)
 {
     __shared__ uint32_t wsum[17];
     __shared__ uint32_t s_tile, s_prefix;
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(1024) void k_scan_chained(uint32_t *__restrict__ da
     unsigned long long *word = state + 1;
     const uint32_t idx = tile * (uint32_t) kScanTile + (uint32_t) threadIdx.x * 4u;
     uint32_t v[4], total;
-    const uint32_t local = block_exclusive_scan(load4(data, idx, m, v), wsum, total);
+    const uint32_t local = block_exclusive_scan(load4(in, idx, m, v), wsum, total);
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         uint32_t prefix = 0;
@@ -403,7 +403,8 @@ __global__ __launch_bounds__(1024) void k_scan_chained(uint32_t *__restrict__ da
             // window: predecessors top - lane, top = the nearest one not yet summed
             int top = (int) tile - 1;
             uint32_t polls = 0;
-            while (top >= 0) {
+            bool raw = poll_limit == 0;
+            while (!raw && top >= 0) {
                 const int k = top - lane;
                 const unsigned long long w =
                     k >= 0 ? __hip_atomic_load(&word[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ull << 32);
@@ -413,7 +414,10 @@ __global__ __launch_bounds__(1024) void k_scan_chained(uint32_t *__restrict__ da
                 const int stop = incl ? __builtin_ctzll(incl) : 63;
                 const unsigned long long need = stop == 63 ? ~0ull : ((2ull << stop) - 1ull);
                 if (none & need) {   // a predecessor in the window has published nothing yet
-                    if (++polls >= kScanSpinLimit) break;
+                    if (++polls >= poll_limit) {
+                        raw = true;
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
@@ -423,6 +427,14 @@ __global__ __launch_bounds__(1024) void k_scan_chained(uint32_t *__restrict__ da
                 prefix += x;
                 if (incl) break;
                 top -= 64;
+            }
+            if (raw) {   // tiles 0..top, from their raw values
+                uint32_t x = 0;
+                const uint32_t end = (uint32_t) (top + 1) * (uint32_t) kScanTile;
+                for (uint32_t q = (uint32_t) lane; q < end; q += 64u) x += in[q];
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+                prefix += x;
             }
             if (lane == 0)
                 __hip_atomic_store(&word[tile], (2ull << 32) | (prefix + total), __ATOMIC_RELAXED,
@@ -434,7 +446,7 @@ __global__ __launch_bounds__(1024) void k_scan_chained(uint32_t *__restrict__ da
     uint32_t run = s_prefix + local;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (idx + q < m) data[idx + q] = run;
+        if (idx + q < m) out[idx + q] = run;
         run += v[q];
     }
 }
@@ -1807,13 +1819,15 @@ void launch_hash(const ParticleSoA &p, uint32_t *key, const SceneTable &t, doubl
 
 void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out, uint32_t *vals_out,
                        const SortTile *tiles, uint32_t ntiles, int shift, int bits, uint32_t *hist, uint32_t *partials,
-                       hipStream_t st)
+                       hipStream_t st, uint32_t poll_limit)
 {
     if (ntiles == 0) return;
     const uint32_t m = ntiles << bits;
     const uint32_t sb = scan_blocks(m);
-    // more than one scan tile: one single-pass scan, its state in `partials` (zeroed by the count kernel)
-    unsigned long long *state = sb > 1 ? reinterpret_cast<unsigned long long *>(partials) : nullptr;
+    // more than one scan tile: one single-pass scan from hist into partials[0, m), its state after that (zeroed
+    // by the count kernel; m is a multiple of 256, so the state is 8-byte aligned)
+    uint32_t *offsets = sb > 1 ? partials : hist;
+    unsigned long long *state = sb > 1 ? reinterpret_cast<unsigned long long *>(partials + m) : nullptr;
     const uint32_t nstate = sb > 1 ? sb + 1 : 0u;
     if (bits == kRadixBitsWide)
         hipLaunchKernelGGL(k_radix_count<kRadixBitsWide>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, tiles,
@@ -1824,13 +1838,14 @@ void launch_radix_pass(const uint32_t *keys_in, const uint32_t *vals_in, uint32_
     if (sb == 1)
         hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, st, hist, m);
     else
-        hipLaunchKernelGGL(k_scan_chained, dim3(sb), dim3(1024), 0, st, hist, m, state);
+        hipLaunchKernelGGL(k_scan_chained, dim3(sb), dim3(1024), 0, st, (const uint32_t *) hist, offsets, m, state,
+                           poll_limit);
     if (bits == kRadixBitsWide)
         hipLaunchKernelGGL(k_radix_scatter<kRadixBitsWide>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in,
-                           keys_out, vals_out, tiles, shift, (const uint32_t *) hist);
+                           keys_out, vals_out, tiles, shift, (const uint32_t *) offsets);
     else
         hipLaunchKernelGGL(k_radix_scatter<kRadixBits>, dim3(ntiles), dim3(kSortThreads), 0, st, keys_in, vals_in,
-                           keys_out, vals_out, tiles, shift, (const uint32_t *) hist);
+                           keys_out, vals_out, tiles, shift, (const uint32_t *) offsets);
 }
 
 void launch_exclusive_scan(uint32_t *data, uint32_t m, uint32_t *partials, hipStream_t st)

@@ -386,6 +386,37 @@ def test_sort_across_key_widths(hip, orc, res, n):
     sim.close()
 
 
+@pytest.mark.parametrize("limit", ["0", "1"])
+def test_scan_fallback_is_exact(hip, orc, monkeypatch, limit):
+    """The histogram scan's bounded wait: with SPH_SCAN_POLL_LIMIT=0 every scan
+    tile adds up its predecessors' raw counts itself, with 1 a tile does so as
+    soon as a predecessor has not published; the sort (three 10-bit passes, 13
+    scan tiles each) is still a stable argsort, and 5 steps of a 110,592-particle
+    dam break in the 3.2 m box (two 10-bit passes, 7 scan tiles each) are bitwise
+    those of a context with the default limit."""
+    monkeypatch.setenv("SPH_SCAN_POLL_LIMIT", limit)
+    H = hip.scenes.H
+    res, n = 257, 200_000
+    box = float(np.float32((res - 0.5) * float(H)))
+    rng = np.random.default_rng(5)
+    pos = ((rng.random((n, 3)) - 0.5) * box).astype(np.float32)
+    sim = gpu_sim(hip, box, pos, diag=False)
+    sim.updateGrid()
+    pre = sim.debug(hip._lib.DEBUG_CELL_PRESORT)
+    np.testing.assert_array_equal(sim.debug(hip._lib.DEBUG_PERM), np.argsort(pre, kind="stable").astype(np.uint32))
+    sim.close()
+    lat = hip.scenes.lattice(48, 48, 48, 3.2, jitter_h=0.01, seed=3)   # 71^3 cells: 19-bit keys
+    a = gpu_sim(hip, 3.2, lat, diag=False)
+    monkeypatch.delenv("SPH_SCAN_POLL_LIMIT")
+    b = gpu_sim(hip, 3.2, lat, diag=False)
+    a.advance(5)
+    b.advance(5)
+    pa, va, _ = a.download()
+    pb, vb, _ = b.download()
+    assert_bits_equal(pa, pb, "positions, limit %s vs default" % limit)
+    assert_bits_equal(va, vb, "velocities, limit %s vs default" % limit)
+
+
 def test_full_workloads_stay_whole_finite_and_in_the_box(hip):
     """The benchmark workloads at full size and length: the 1M dam break for 1,000
     steps and the fountain to 2,000 steps (524,288 particles, graph replay). Every
