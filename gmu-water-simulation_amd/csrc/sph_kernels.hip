@@ -1197,11 +1197,12 @@ __device__ __forceinline__ PairTerms pair_terms(const BufView &bv, const Consts 
 {
     const float dx = xi - n.x, dyv = yi - n.y, dzv = zi - n.z;
     const float r2 = dx * dx + dyv * dyv + dzv * dzv;
-    const double r = sqrt_of_float((double) r2);
+    const double x2 = (double) r2, y2 = __builtin_amdgcn_rsq(x2);
+    const double r = sqrt_with_rsq(x2, y2);
     const double hr = k.h - r;
     const float sg = spiky_scale(hr, r2, k);
     float gx, gy, gz;
-    div3_small_divisor(dx * sg, dyv * sg, dzv * sg, (float) r, gx, gy, gz);
+    div3_small_divisor(dx * sg, dyv * sg, dzv * sg, (float) r, gx, gy, gz, y2);
     const float pf = (float) (ai + n.prr);
     PairTerms t;
     t.px = gx * pf;

@@ -137,9 +137,8 @@ __device__ __forceinline__ float spiky_factor(double hr, float r2, double spiky,
 // [0, h^2]: checked against sqrt() for each of them on the device (a third
 // fewer FP64 operations than the compiler's Goldschmidt sequence for a double
 // sqrt, which this replaced after the same check).
-__device__ __forceinline__ double sqrt_of_float(double x)
+__device__ __forceinline__ double sqrt_with_rsq(double x, double y)
 {
-    const double y = __builtin_amdgcn_rsq(x);
     const double hh = y * 0.5;
     double g = x * y;
     double d = __builtin_fma(-g, g, x);
@@ -149,12 +148,19 @@ __device__ __forceinline__ double sqrt_of_float(double x)
     return x == 0.0 ? 0.0 : g;
 }
 
-// 1/b to within one rounding: hardware estimate plus two Newton-Raphson steps.
-__device__ __forceinline__ double rcp_refined(double b)
+__device__ __forceinline__ double sqrt_of_float(double x)
 {
-    double r = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
+    return sqrt_with_rsq(x, __builtin_amdgcn_rsq(x));
+}
+
+// 1/b from an estimate y of it good to about 2^-22 (relative): two
+// Newton-Raphson steps. The force kernels pass rsq(r^2) for b = (float) r, which
+// is within about 2^-23 of 1/b; a device test checks |1 - b * rb| <= 2^-52 for
+// every float r^2 in (0, h^2]. (After one step it is about 2^-46.)
+__device__ __forceinline__ double rcp_from(double b, double y)
+{
+    double e = __builtin_fma(-b, y, 1.0);
+    double r = __builtin_fma(y, e, y);
     e = __builtin_fma(-b, r, 1.0);
     return __builtin_fma(r, e, r);
 }
@@ -190,9 +196,9 @@ __device__ __forceinline__ bool all_normal3(float q0, float q1, float q2)
 // a_c / b for b in (0, 1): |a/b| > |a| >= 2^-149 for a != 0, so a zero result
 // means a = 0 (exact); only subnormal results take the IEEE division.
 __device__ __forceinline__ void div3_small_divisor(float a0, float a1, float a2, float b, float &q0, float &q1,
-                                                  float &q2)
+                                                  float &q2, double est)
 {
-    const double rb = rcp_refined((double) b);
+    const double rb = rcp_from((double) b, est);
     q0 = (float) ((double) a0 * rb);
     q1 = (float) ((double) a1 * rb);
     q2 = (float) ((double) a2 * rb);

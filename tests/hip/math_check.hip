@@ -44,7 +44,10 @@ __device__ void report(uint32_t local, uint64_t first_local, unsigned long long 
     }
 }
 
-// sqrt_of_float((double) f) == sqrt((double) f) for the float bit patterns lo .. lo+count-1.
+// For the float bit patterns lo .. lo+count-1 of r^2: sqrt_of_float((double) f)
+// == sqrt((double) f), and (f > 0) the reciprocal of the pressure gradient's
+// divisor b = (float) sqrt, formed from the same rsq estimate as in the force
+// kernels (rcp_from), is within 2^-52 of 1/b: |1 - b * rb| <= 2^-52.
 __global__ __launch_bounds__(256) void k_sqrt(uint32_t lo, uint32_t count, unsigned long long *bad,
                                               unsigned long long *first)
 {
@@ -52,8 +55,16 @@ __global__ __launch_bounds__(256) void k_sqrt(uint32_t lo, uint32_t count, unsig
     uint64_t fl = ~0ull;
     for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < count; t += gridDim.x * 256u) {
         const double x = (double) __uint_as_float(lo + t);
-        const double a = sph::sqrt_of_float(x), b = sqrt(x);
-        if (__double_as_longlong(a) != __double_as_longlong(b)) {
+        const double y = __builtin_amdgcn_rsq(x);
+        const double a = sph::sqrt_with_rsq(x, y), b = sqrt(x);
+        bool ok = __double_as_longlong(a) == __double_as_longlong(b) &&
+                  __double_as_longlong(sph::sqrt_of_float(x)) == __double_as_longlong(b);
+        if (x > 0.0) {
+            const double bf = (double) (float) a;
+            const double e = __builtin_fma(-bf, sph::rcp_from(bf, y), 1.0);
+            ok = ok && __builtin_fabs(e) <= 0x1p-52;
+        }
+        if (!ok) {
             ++local;
             fl = fl < (uint64_t) (lo + t) ? fl : (uint64_t) (lo + t);
         }
@@ -72,9 +83,11 @@ __device__ __forceinline__ float random_float(uint64_t r, int emin, int emax)
 }
 
 // The two quotient helpers against the IEEE float division, on n random cases:
-// div3_small_divisor for divisors in [2^-20, h] and div3_rcp (1/b in double) for
-// divisors in [2^-20, 2^20], with numerators from 2^-149 to 2^10 so that many
-// quotients round to subnormals, to zero and to the smallest normals.
+// div3_small_divisor for the divisors of the pressure gradient (b = (float)
+// sqrt(r^2) for a random float r^2 in [2^-40, h^2], with the force kernels' rsq
+// estimate) and div3_rcp (1/b in double) for divisors in [2^-20, 2^20], with
+// numerators from 2^-149 to 2^10 so that many quotients round to subnormals, to
+// zero and to the smallest normals.
 __global__ __launch_bounds__(256) void k_div(uint64_t seed, uint64_t n, unsigned long long *bad,
                                              unsigned long long *first)
 {
@@ -85,10 +98,12 @@ __global__ __launch_bounds__(256) void k_div(uint64_t seed, uint64_t n, unsigned
         const float a0 = random_float(mix64(s + 1), -149, 10);
         const float a1 = random_float(mix64(s + 2), -130, -100);
         const float a2 = (mix64(s + 3) & 15) == 0 ? 0.0f : random_float(mix64(s + 4), -149, -120);
-        const float bs = fabsf(random_float(mix64(s + 5), -20, -5));
-        const float b = bs < 0.0457f ? bs : 0.0457f;
+        const float r2s = fabsf(random_float(mix64(s + 5), -40, -9));
+        const float r2 = r2s < 0.0457f * 0.0457f ? r2s : 0.0457f * 0.0457f;
+        const double x = (double) r2, y = __builtin_amdgcn_rsq(x);
+        const float b = (float) sph::sqrt_with_rsq(x, y);
         float q0, q1, q2;
-        sph::div3_small_divisor(a0, a1, a2, b, q0, q1, q2);
+        sph::div3_small_divisor(a0, a1, a2, b, q0, q1, q2, y);
         bool ok = __float_as_uint(q0) == __float_as_uint(a0 / b) && __float_as_uint(q1) == __float_as_uint(a1 / b) &&
                   __float_as_uint(q2) == __float_as_uint(a2 / b);
         const float c = fabsf(random_float(mix64(s + 6), -20, 20));

@@ -4,6 +4,8 @@
   * sqrt_of_float, the force pass's double sqrt of r^2 (src/CCPUParticleSimulator.cpp:20),
     equals the compiler's correctly rounded sqrt for EVERY float r^2 in [0, h^2]
     (all 990,437,133 bit patterns): the whole input domain of an accepted pair;
+    on the same inputs, the reciprocal of the pressure gradient's divisor
+    (float) r, formed from the sqrt's rsq estimate, is within 2^-52 of 1/(float) r;
   * the kernels' library powers (the correctly rounded pow(h^2 - r^2, 3) and
     pow(h - r, 2), corrected by the generated tables where glibc's pow differs,
     src/CCPUParticleSimulator.cpp:14, 21) equal this host's glibc pow for EVERY
