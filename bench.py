@@ -108,11 +108,13 @@ def load_traffic(workload):
         return None
 
 
-def bench_batched(sphhip, steps, per_scene_steps=None, nscenes=512, seed=3):
+def bench_batched(sphhip, steps, per_scene_steps=None, nscenes=512, seed=3, variant=None):
     """The batched workload: one context for all scenes vs one context per scene.
     Both time steps 4 onwards of the same scenes (the autotuner restores the
     state it stepped); per_scene_steps None times the per-scene run over the
-    same window as the batched one."""
+    same window as the batched one. variant: a kernel shape (row masks) to use
+    instead of the autotuner's choice, for profiling runs that must compare
+    like with like."""
     from sphhip import _lib
     lib = _lib.load()
     w = sphhip.scenes.batch_workload(nscenes=nscenes, seed=seed)
@@ -121,7 +123,11 @@ def bench_batched(sphhip, steps, per_scene_steps=None, nscenes=512, seed=3):
     for i, s in enumerate(w):
         b.upload(i, s["pos"])
     b.step(2)
-    tune = b.autotune(steps=3)
+    if variant is None:
+        tune = b.autotune(steps=3)
+    else:   # the autotuner leaves the state at step 2 as well, so the same window is timed
+        b.set_variant(variant, 2)
+        tune = {}
     variant, mode = b.variant()
     b.set_graphs(True)
     b.step(2)   # captures the graphs of both buffer parities (untimed)
@@ -291,6 +297,8 @@ def main():
                     help="steps timed from step 200 on as well (0 skips; only when the main window ends by 200)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=500, help="0 skips the batched measurement")
+    ap.add_argument("--batched-variant", type=int, default=None,
+                    help="kernel shape of the batched step (row masks) instead of the autotuner's choice")
     ap.add_argument("--per-scene-steps", type=int, default=None,
                     help="steps of the per-scene comparison (default: --batched-steps; 0 skips it)")
     ap.add_argument("--fountain-steps", type=int, default=2000, help="0 skips the fountain measurement")
@@ -319,7 +327,7 @@ def main():
     if args.only:
         print("bench: only the %s part" % args.only, file=sys.stderr, flush=True)
         if args.only == "batched":
-            part = bench_batched(sphhip, args.batched_steps, args.per_scene_steps)
+            part = bench_batched(sphhip, args.batched_steps, args.per_scene_steps, variant=args.batched_variant)
         elif args.only == "fountain":
             part = bench_fountain(sphhip, args.fountain_steps, min(512, args.fountain_steps))
         else:
@@ -429,7 +437,8 @@ def main():
     sim.close()
     if args.batched_steps:
         print("bench: batched workload, %d steps" % args.batched_steps, file=sys.stderr, flush=True)
-        line["batched"] = bench_batched(sphhip, args.batched_steps, args.per_scene_steps)
+        line["batched"] = bench_batched(sphhip, args.batched_steps, args.per_scene_steps,
+                                        variant=args.batched_variant)
     if args.fountain_steps:
         print("bench: fountain workload, %d steps" % args.fountain_steps, file=sys.stderr, flush=True)
         line["fountain"] = bench_fountain(sphhip, args.fountain_steps, min(512, args.fountain_steps))
